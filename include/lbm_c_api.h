@@ -1,0 +1,274 @@
+/*
+ * C ABI of the MI355X lattice-Boltzmann collide-stream path.
+ *
+ * The reference (Resacd/Reza-Seti) has no FFI; its "operator API" is the C++
+ * class surface of Simulation / SubDomain / Geometry / Boundary /
+ * LatticeModel. Each entry point below names the reference interface it
+ * replaces. Plain pointers and sizes only; no C++ or torch types cross this
+ * boundary; no C++ exception escapes it. Every call returns LBM_OK (0) or an
+ * LBM_ERR_* code, and lbm_last_error() returns the calling thread's message.
+ *
+ * Node order: every per-node array is in the reference's node order, i.e. the
+ * order of SubDomain::Involved_Lat_ (Subdomain.cpp:598-611: grid index i
+ * outermost, k innermost, fluid points only). Populations are node-major,
+ * f[n * Q + q], with the reference's q order (LatticeModel.cpp:92-191 for
+ * D3Q19, :249-374 for D3Q27).
+ *
+ * A solver handle owns one HIP stream; calls on one handle are not
+ * re-entrant, distinct handles are independent.
+ */
+#ifndef LBM_C_API_H
+#define LBM_C_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_ERR_ARG 1          /* invalid argument or inconsistent sizes */
+#define LBM_ERR_IO 2           /* config / STL could not be read */
+#define LBM_ERR_HIP 3          /* HIP runtime error (no device, OOM, launch) */
+#define LBM_ERR_UNSUPPORTED 4  /* mode not available for this lattice/precision */
+#define LBM_ERR_PARITY 5       /* a reference-mode assumption was violated (see lbm_step) */
+#define LBM_ERR_STATE 6        /* call out of order (e.g. step before create) */
+
+/* Node kinds (Struct.h:53-67 BoundaryType, plus plain fluid). */
+#define LBM_KIND_FLUID 0
+#define LBM_KIND_WALL 1
+#define LBM_KIND_INLET 2
+#define LBM_KIND_OUTLET 3
+
+/* Population storage / arithmetic. */
+#define LBM_PREC_F64 0          /* f stored and computed in FP64, unshifted (reference parity) */
+#define LBM_PREC_F32_SHIFTED 1  /* f - w stored in FP32, FP32 collision */
+#define LBM_PREC_F16_SHIFTED 2  /* f - w stored in FP16, FP32 collision */
+
+/* Wall treatment. */
+#define LBM_WALL_REFERENCE 0    /* Simulation::ApplyWallBc copy rule (Simulation.cpp:746-763) */
+#define LBM_WALL_HALFWAY_BB 1   /* Node::BounceBack half-way rule on every missing link (Node.cpp:144-148) */
+
+/* Inlet treatment. */
+#define LBM_INLET_REFERENCE_ZOUHE 0  /* Boundary::ApplyBc (Boundary.cpp:111-289), D3Q19 only */
+#define LBM_INLET_NEE_VELOCITY 1     /* non-equilibrium extrapolation, per-node velocity profile */
+#define LBM_INLET_NONE 2             /* inlet nodes behave as plain fluid nodes */
+
+/* Outlet treatment. */
+#define LBM_OUTLET_REFERENCE 0       /* Simulation::ApplyOutletBc copy (Simulation.cpp:650-741) */
+#define LBM_OUTLET_NEE_PRESSURE 1    /* non-equilibrium extrapolation at fixed density */
+#define LBM_OUTLET_NONE 2
+
+/* Inlet speed source for lbm_step. */
+#define LBM_FLOW_CONSTANT 0   /* CteVel every step (Simulation.cpp:558-559) */
+#define LBM_FLOW_PULSATILE 1  /* Simulation::getInletVelocity(t, 0.9) (Simulation.cpp:519-534, pulsatile.h) */
+
+/* Lattice tables: VelocitySet::GetDirection / GetWeight and
+ * LatticeModel::OppositeDirection (VelocitySet.h:60-65, LatticeModel.h:50-54).
+ * c: 3 ints per direction; q is 19 or 27. */
+int lbm_lattice(int32_t q, int32_t* c, int32_t* opp, double* w);
+
+/* ------------------------------------------------------------------------ */
+/* Configuration                                                             */
+/* ------------------------------------------------------------------------ */
+
+#define LBM_MAX_BLOCKS 16
+
+typedef struct {
+  char plane[16];
+  double coordinate;
+  double range_x[2], range_y[2], range_z[2];
+  double normal[3];
+  int32_t normal_len;
+} lbm_plane_block;
+
+/* Mirrors struct SimulationParameters (SimulationParameters.h:49-89). */
+typedef struct {
+  char stl_file[512];
+  char flow_type[32];
+  double dt, rho, nu_blood, tau, dx, pressure, reynolds_number, cte_vel;
+  double systole, diastole;
+  int32_t num_out, num_cycle, max_iter, iter_to_write;
+  int32_t n_inlets, n_outlets, n_equations;
+  lbm_plane_block inlets[LBM_MAX_BLOCKS];
+  lbm_plane_block outlets[LBM_MAX_BLOCKS];
+} lbm_config;
+
+/* Replaces parseInputFile (LBM.h:81-188). LBM_ERR_IO if the file cannot be
+ * opened; LBM_ERR_ARG on a numeric conversion error (the reference throws). */
+int lbm_config_parse(const char* path, lbm_config* out);
+
+/* ------------------------------------------------------------------------ */
+/* Geometry and node tables (host)                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct lbm_geometry lbm_geometry;
+
+typedef struct {
+  int32_t q;               /* 19 or 27 */
+  int32_t periodic[3];     /* extension: periodic axes (reference: none) */
+  int32_t nthreads;        /* host threads, 0 = all */
+  int32_t verify_stride;   /* >0: re-check every n-th column against all triangles */
+  int32_t compute_normals; /* near-wall normals (Subdomain.cpp:499-572) */
+} lbm_geometry_options;
+
+void lbm_geometry_options_default(lbm_geometry_options* o);
+
+/* Replaces the Geometry + SubDomain constructors (ComplexGeometry.cpp:48-73,
+ * Subdomain.cpp:43-62) for one rank: STL -> grid -> ray labels -> boundary
+ * sets -> nodes -> connectivity -> near-wall set and normals. The STL path in
+ * the config is taken relative to the config file's directory. */
+int lbm_geometry_from_config(const char* config_path, const lbm_geometry_options* opts,
+                             lbm_geometry** out);
+
+/* Same chain starting from a fluid label per grid point (uint8, index
+ * (i*ny + j)*nz + k), skipping STL reading and ray casting. origin/spacing
+ * give point coordinates origin + index*spacing (spacing is the lattice
+ * pitch itself, not config grid_size). tris (12 doubles per triangle:
+ * 3 vertices then file normal) is only used for near-wall normals and may be
+ * NULL with ntris = 0. */
+int lbm_geometry_from_labels(const uint8_t* fluid, int32_t nx, int32_t ny, int32_t nz,
+                             const double origin[3], double spacing,
+                             const lbm_plane_block* inlets, int32_t n_inlets,
+                             const lbm_plane_block* outlets, int32_t n_outlets,
+                             const double* tris, int64_t ntris,
+                             const lbm_geometry_options* opts, lbm_geometry** out);
+
+typedef struct {
+  int64_t n_nodes, n_fluid, n_wall, n_inlet, n_outlet, n_nearwall, n_boundary_points;
+  int64_t n_triangles, upstream_out_of_box;
+  int32_t grid[3];
+  int32_t q;
+  int32_t n_outlets;
+  int32_t has_inlet_normal;
+  double spacing;
+  double bbox[6];          /* min_x, max_x, min_y, max_y, min_z, max_z */
+  double inlet_normal[3];
+} lbm_geometry_info;
+
+int lbm_geometry_get_info(const lbm_geometry* g, lbm_geometry_info* out);
+/* Grid labels (Point::Type Fluid/NotFluid after labelPoints). */
+int lbm_geometry_get_labels(const lbm_geometry* g, uint8_t* fluid);
+/* ijk: 3 per node; kind: LBM_KIND_*; outlet_id: outlet block or -1. Any may be NULL. */
+int lbm_geometry_get_nodes(const lbm_geometry* g, int32_t* ijk, uint8_t* kind, int32_t* outlet_id);
+/* Push neighbour table Node::m_neighbours (Subdomain.cpp:447-471): nb[q*N + n]
+ * = node index at x + c_q, or -1. */
+int lbm_geometry_get_neighbours(const lbm_geometry* g, int32_t* nb);
+/* Near-wall nodes (ascending node index) and their normals (3 per node). */
+int lbm_geometry_get_nearwall(const lbm_geometry* g, int32_t* nodes, double* normals);
+/* Node behind each inlet/outlet node along its block normal, or -1. */
+int lbm_geometry_get_bc_neighbour(const lbm_geometry* g, int32_t* bc_nbr);
+/* Outlet block normals (3 per outlet). */
+int lbm_geometry_get_outlet_normals(const lbm_geometry* g, double* normals);
+void lbm_geometry_destroy(lbm_geometry* g);
+
+/* ------------------------------------------------------------------------ */
+/* Solver (device)                                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct lbm_solver lbm_solver;
+
+typedef struct {
+  int32_t precision;       /* LBM_PREC_* */
+  int32_t wall_mode;       /* LBM_WALL_* */
+  int32_t inlet_mode;      /* LBM_INLET_* */
+  int32_t outlet_mode;     /* LBM_OUTLET_* */
+  int32_t flow_type;       /* LBM_FLOW_* */
+  int32_t wss_every;       /* accumulate WSS every n-th step (reference: 1); 0 = never */
+  int32_t parity_checks;   /* FP64 reference mode: detect the f*==0 streaming skip */
+  int32_t device;          /* HIP device ordinal */
+  double tau;              /* relaxation time (Simulation::SetRelaxation) */
+  double dt;               /* time step (physical seconds) */
+  double cte_vel;          /* inlet speed, lattice units */
+  double spacing;          /* lattice pitch used in the WSS constant (0 = from geometry) */
+  double outlet_rho;       /* LBM_OUTLET_NEE_PRESSURE density */
+  double force[3];         /* extension: constant body-force density (Guo forcing) */
+} lbm_params;
+
+/* Reference defaults: FP64, reference walls/inlet/outlet, constant flow,
+ * WSS every step, parity checks on, device 0, outlet_rho 1. */
+void lbm_params_default(lbm_params* p);
+
+/* Replaces the state set-up of the Simulation constructor
+ * (Simulation.cpp:44-58) and SubDomain::Initialize (Subdomain.cpp:578-594):
+ * populations f = w_q, accumulators zero, and the first collision done. */
+int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out);
+
+/* Per-inlet-node velocity profile: u_in(node, t) = sum_m coef_m(t) * basis[m][i]
+ * for inlet node i (creation order). Default: one mode, basis 1, coef = the
+ * lbm_step inlet speed (the reference's uniform plug). n_modes <= 16. */
+int lbm_set_inlet_profile(lbm_solver* s, int32_t n_modes, const double* basis);
+
+/* Extension: Brinkman damping field (>= 0) per node; NULL clears it. */
+int lbm_set_porosity(lbm_solver* s, const float* alpha);
+
+/* Advance nsteps reference iterations (Simulation::Iterate loop body,
+ * Simulation.cpp:553-568: Collision, Communicate, Streaming, inlet speed,
+ * ApplyZouHeBc, ApplyOutletBc, ApplyWallBc, ComputeWallShearStress), each one
+ * fused into a single kernel launch. The inlet speed follows params.flow_type
+ * with the reference's accumulated time (t += dt per step).
+ * In FP64 reference mode with parity_checks, returns LBM_ERR_PARITY if the
+ * f*==0.0 streaming skip (Simulation.cpp:245-248) hit a node whose previous
+ * populations are not kept (a plain fluid node), which would break parity. */
+int lbm_step(lbm_solver* s, int32_t nsteps);
+
+/* Same, with explicit profile coefficients per step: coef[t * n_modes + m]. */
+int lbm_step_coefficients(lbm_solver* s, int32_t nsteps, const double* coef);
+
+/* Post-BC populations f(t) (Node::m_distributions after the last iteration),
+ * FP64, node-major, reference node order. */
+int lbm_get_populations(lbm_solver* s, double* f);
+/* Post-collision populations of the next iteration (m_newDistributions of the
+ * collision that follows), FP64, same layout. */
+int lbm_get_post_collision(lbm_solver* s, double* fs);
+/* Overwrite f(t) of every node (Node::m_distributions) and redo the collision. */
+int lbm_set_populations(lbm_solver* s, const double* f);
+
+/* Near-wall WSS in lbm_geometry_get_nearwall order: instantaneous |tau|
+ * (Node::wall_shear_stress), accumulated tau (3 per node) and accumulated
+ * |tau| (Simulation.cpp:767-799). Any pointer may be NULL. */
+int lbm_get_wss(lbm_solver* s, double* wss, double* acc_tau, double* acc_mag);
+
+/* TAWSS and OSI from the accumulators (Simulation::ComputeOSIAndTAWSS,
+ * Simulation.cpp:804-826). */
+int lbm_get_osi_tawss(lbm_solver* s, double* osi, double* tawss);
+
+/* Density and velocity of f(t) per node (the values Simulation::Collision
+ * computes at the start of the next iteration; wall velocity is 0 in
+ * reference wall mode). */
+int lbm_get_macroscopic(lbm_solver* s, double* rho, double* u);
+
+typedef struct {
+  double mass;             /* sum over nodes of rho, FP64 accumulation */
+  double momentum[3];      /* sum of rho*u */
+  double wss_l2;           /* sqrt(sum of instantaneous |tau|^2) over near-wall nodes */
+  double wss_max;
+  int64_t iteration;
+  double time;             /* reference total_runtime */
+} lbm_stats;
+
+/* Wavefront-reduction kernel over the current post-collision state. */
+int lbm_reduce(lbm_solver* s, lbm_stats* out);
+
+/* Run nsteps and report the device time of the step kernels, measured with
+ * HIP events on the solver's stream (milliseconds, all steps). */
+int lbm_step_timed(lbm_solver* s, int32_t nsteps, double* ms);
+
+int lbm_synchronize(lbm_solver* s);
+void lbm_destroy(lbm_solver* s);
+
+typedef struct {
+  int64_t n_nodes, n_special, n_bulk, n_pad;
+  int64_t device_bytes;
+  int32_t q, precision;
+  double bytes_per_update;  /* algorithmic: 2 * Q * sizeof(population) */
+} lbm_solver_info;
+int lbm_get_solver_info(const lbm_solver* s, lbm_solver_info* out);
+
+const char* lbm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_C_API_H */

@@ -1,0 +1,732 @@
+// Fused collide-stream kernels for gfx950 (MI355X).
+//
+// One launch = one reference iteration's Streaming + inlet/outlet/wall
+// boundary rules + wall shear stress (Simulation.cpp:555-568) followed by the
+// next iteration's Collision (Simulation.cpp:77-232). The state between
+// launches is the post-collision population f* in SoA layout over a sparse
+// slot list; streaming is a pull through an int32 table.
+//
+// Slots are split in two ranges:
+//   special [0, n_special): wall, inlet, outlet and near-wall nodes. They
+//     carry side data (previous post-BC f for missing links, boundary
+//     neighbour, WSS accumulators) and run the full boundary logic.
+//   bulk [n_special_pad, bulk_end): plain fluid nodes with every link
+//     present: pull Q values, collide, store Q values.
+// Special blocks are dispatched first (they are the heavier ones); bulk
+// blocks are remapped so that each XCD walks a contiguous slot range and
+// neighbour reads stay in that XCD's L2.
+//
+// FP64 (precision 0) reproduces the reference arithmetic term by term; the
+// library is built with -ffp-contract=off so no FMA is formed. The FP32/FP16
+// paths store f - w_q and compute in FP32 with contraction enabled locally.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "../lattice.h"
+#include "kernels.h"
+
+namespace lbm {
+
+namespace {
+
+constexpr int WALL_REF = 0, WALL_BB = 1;
+constexpr int INLET_ZOUHE = 0, INLET_NEE = 1;
+constexpr int OUTLET_REF = 0, OUTLET_NEE = 1;
+
+// ---------------------------------------------------------------- storage
+template <class S>
+struct Pop;
+template <>
+struct Pop<double> {
+  using C = double;
+  static constexpr bool shifted = false;
+  __device__ static C ld(const void* base, int64_t i) { return static_cast<const double*>(base)[i]; }
+  __device__ static void st(void* base, int64_t i, C v) { static_cast<double*>(base)[i] = v; }
+};
+template <>
+struct Pop<float> {
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const void* base, int64_t i) { return static_cast<const float*>(base)[i]; }
+  __device__ static void st(void* base, int64_t i, C v) { static_cast<float*>(base)[i] = v; }
+};
+template <>
+struct Pop<__half> {
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const void* base, int64_t i) {
+    return __half2float(static_cast<const __half*>(base)[i]);
+  }
+  __device__ static void st(void* base, int64_t i, C v) { static_cast<__half*>(base)[i] = __float2half_rn(v); }
+};
+
+// ------------------------------------------------- reference FP64 collision
+// Velocity sums grouped exactly as Simulation.cpp:100-134 (D3Q19) and
+// :149-207 (D3Q27); each line is a left-to-right sum divided by rho.
+template <class L>
+struct RefVelocity;
+template <>
+struct RefVelocity<D3Q19> {
+  __device__ static void get(const double* f, double rho, double& ux, double& uy, double& uz) {
+    ux = ((f[0] + f[6] + f[7] + f[8] + f[9]) - (f[1] + f[10] + f[11] + f[12] + f[13])) / rho;
+    uy = ((f[2] + f[6] + f[10] + f[14] + f[15]) - (f[3] + f[7] + f[11] + f[16] + f[17])) / rho;
+    uz = ((f[4] + f[8] + f[12] + f[14] + f[16]) - (f[5] + f[9] + f[13] + f[15] + f[17])) / rho;
+  }
+};
+template <>
+struct RefVelocity<D3Q27> {
+  __device__ static void get(const double* f, double rho, double& ux, double& uy, double& uz) {
+    ux = ((f[1] + f[7] + f[8] + f[11] + f[13] + f[19] + f[21] + f[23] + f[25]) -
+          (f[2] + f[9] + f[10] + f[12] + f[14] + f[20] + f[22] + f[24] + f[26])) / rho;
+    uy = ((f[3] + f[7] + f[9] + f[15] + f[16] + f[19] + f[21] + f[24] + f[26]) -
+          (f[4] + f[8] + f[10] + f[17] + f[18] + f[20] + f[22] + f[23] + f[25])) / rho;
+    uz = ((f[5] + f[11] + f[12] + f[15] + f[17] + f[19] + f[22] + f[23] + f[26]) -
+          (f[6] + f[13] + f[14] + f[16] + f[18] + f[20] + f[21] + f[24] + f[25])) / rho;
+  }
+};
+
+// m_eqdistributions[q] of Simulation.cpp:214-224.
+template <class L>
+__device__ inline double feq_ref(int q, double rho, double ux, double uy, double uz) {
+  const double du = ux * L::cx[q] + uy * L::cy[q] + uz * L::cz[q];
+  const double u_sqr = ux * ux + uy * uy + uz * uz;
+  return L::w[q] * rho * (1.0 + 3.0 * du + 4.5 * du * du - 1.5 * u_sqr);
+}
+
+// Simulation::Collision for one node (Simulation.cpp:79-231).
+template <class L>
+__device__ inline void collide_ref(double* f, bool wall, double omega, double& rho, double& ux, double& uy,
+                                   double& uz) {
+  rho = 0.0;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) rho += f[q];
+  if (wall) {
+    ux = 0.0;
+    uy = 0.0;
+    uz = 0.0;
+  } else {
+    RefVelocity<L>::get(f, rho, ux, uy, uz);
+  }
+  double feq[L::Q];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) feq[q] = feq_ref<L>(q, rho, ux, uy, uz);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) f[q] = f[q] + omega * (feq[q] - f[q]);
+}
+
+// ----------------------------------------- generic collision (extensions)
+// BGK with optional Guo body force and Brinkman damping alpha:
+//   u = (j + F/2) / (rho (1 + alpha/2)),  F_tot = F - alpha rho u,
+//   f* = f + omega (feq - f) + (1 - omega/2) w [3 (c - u).F_tot + 9 (c.u)(c.F_tot)].
+// SHIFT: populations are f - w (then "rho" is accumulated as rho - 1).
+template <class L, class C, bool SHIFT>
+__device__ inline C geq_of(int q, C drho, C rho, C ux, C uy, C uz, C usq) {
+#pragma clang fp contract(fast)
+  const C cu = ux * C(L::cx[q]) + uy * C(L::cy[q]) + uz * C(L::cz[q]);
+  const C wq = C(L::w[q]);
+  const C poly = C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq;
+  if (SHIFT) return wq * (drho + rho * poly);
+  return wq * rho * (C(1) + poly);
+}
+
+template <class L, class C, bool SHIFT>
+__device__ inline void moments(const C* f, C& s, C& jx, C& jy, C& jz) {
+#pragma clang fp contract(fast)
+  s = C(0);
+  jx = C(0);
+  jy = C(0);
+  jz = C(0);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    s += f[q];
+    if (L::cx[q] > 0) jx += f[q];
+    if (L::cx[q] < 0) jx -= f[q];
+    if (L::cy[q] > 0) jy += f[q];
+    if (L::cy[q] < 0) jy -= f[q];
+    if (L::cz[q] > 0) jz += f[q];
+    if (L::cz[q] < 0) jz -= f[q];
+  }
+}
+
+template <class L, class C, bool SHIFT, bool FORCE>
+__device__ inline void collide_ext(C* f, C omega, const C* force, C alpha, bool zero_u, C& drho, C& ux, C& uy,
+                                   C& uz) {
+#pragma clang fp contract(fast)
+  C s, jx, jy, jz;
+  moments<L, C, SHIFT>(f, s, jx, jy, jz);
+  const C rho = SHIFT ? C(1) + s : s;
+  drho = s;
+  C Fx = C(0), Fy = C(0), Fz = C(0);
+  if (FORCE) {
+    const C inv = C(1) / (rho * (C(1) + C(0.5) * alpha));
+    ux = (jx + C(0.5) * force[0]) * inv;
+    uy = (jy + C(0.5) * force[1]) * inv;
+    uz = (jz + C(0.5) * force[2]) * inv;
+    if (zero_u) ux = uy = uz = C(0);
+    Fx = force[0] - alpha * rho * ux;
+    Fy = force[1] - alpha * rho * uy;
+    Fz = force[2] - alpha * rho * uz;
+  } else {
+    const C inv = C(1) / rho;
+    ux = jx * inv;
+    uy = jy * inv;
+    uz = jz * inv;
+    if (zero_u) ux = uy = uz = C(0);
+  }
+  const C usq = ux * ux + uy * uy + uz * uz;
+  const C kf = C(1) - C(0.5) * omega;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const C e = geq_of<L, C, SHIFT>(q, s, rho, ux, uy, uz, usq);
+    C out = f[q] + omega * (e - f[q]);
+    if (FORCE) {
+      const C cx = C(L::cx[q]), cy = C(L::cy[q]), cz = C(L::cz[q]);
+      const C cu = cx * ux + cy * uy + cz * uz;
+      const C cF = cx * Fx + cy * Fy + cz * Fz;
+      const C uF = ux * Fx + uy * Fy + uz * Fz;
+      out += kf * C(L::w[q]) * (C(3) * (cF - uF) + C(9) * cu * cF);
+    }
+    f[q] = out;
+  }
+}
+
+// ------------------------------------------------------------- boundaries
+// Boundary::ApplyBc (Boundary.cpp:111-289): six independent branches on the
+// sign of each normal component, in source order, with the source's
+// expression order. Branches for +-y hard-code the velocity 1.0, as in the
+// reference. SHIFT adds the sum of weights (exactly 1 for every branch) to the
+// density numerator.
+template <class C, bool SHIFT>
+__device__ inline void zouhe_ref(C* m, C vel, const double* normal) {
+  C v0, v1, v2;
+  if (normal[0] > 0) {
+    v0 = vel; v1 = C(0.0); v2 = C(0.0);
+    const C S = m[2] + m[3] + m[4] + m[5] + m[14] + m[15] + m[16] + m[17] + m[18] +
+              C(2) * (m[0] + m[6] + m[7] + m[8] + m[9]);
+    const C rhoN = (SHIFT ? C(1) + S : S) / (C(1) + v0);
+    C Nxy = (C(1.0) / 2) * (m[2] + m[14] + m[15] - m[3] - m[16] - m[17]) - (C(1.0) / 3) * (rhoN * v1);
+    C Nxz = (C(1.0) / 2) * (m[4] + m[16] + m[14] - m[5] - m[15] - m[17]) - (C(1.0) / 3) * (rhoN * v2);
+    m[1] = m[0] - (1 / C(3.0)) * rhoN * v0;
+    m[10] = m[7] + (1 / C(6.0)) * rhoN * (-v0 + v1) - Nxy;
+    m[11] = m[6] + (1 / C(6.0)) * rhoN * (-v0 - v1) + Nxy;
+    m[13] = m[8] + (1 / C(6.0)) * rhoN * (-v0 - v2) + Nxz;
+    m[12] = m[9] + (1 / C(6.0)) * rhoN * (-v0 + v2) - Nxz;
+  }
+  if (normal[0] < 0) {
+    v0 = vel; v1 = C(0.0); v2 = C(0.0);
+    const C S = m[2] + m[3] + m[4] + m[5] + m[14] + m[15] + m[16] + m[17] + m[18] +
+              C(2) * (m[1] + m[10] + m[11] + m[12] + m[13]);
+    const C rhoN = (SHIFT ? C(1) + S : S) / (C(1) - v0);
+    C Nxy = (C(1.0) / 2) * (m[2] + m[14] + m[15] - m[3] - m[16] - m[17]) - (C(1.0) / 3) * (rhoN * v1);
+    C Nxz = (C(1.0) / 2) * (m[4] + m[16] + m[14] - m[5] - m[15] - m[17]) - (C(1.0) / 3) * (rhoN * v2);
+    m[0] = m[1] + (1 / C(3.0)) * rhoN * v0;
+    m[7] = m[10] + (1 / C(6.0)) * rhoN * (v0 - v1) + Nxy;
+    m[6] = m[11] + (1 / C(6.0)) * rhoN * (v0 + v1) - Nxy;
+    m[8] = m[13] + (1 / C(6.0)) * rhoN * (v0 + v2) - Nxz;
+    m[9] = m[12] + (1 / C(6.0)) * rhoN * (v0 - v2) + Nxz;
+  }
+  if (normal[1] > 0) {
+    v0 = C(0.0); v1 = C(1.0); v2 = C(0.0);
+    const C S = m[0] + m[1] + m[4] + m[5] + m[8] + m[9] + m[12] + m[13] + m[18] +
+              C(2) * (m[2] + m[6] + m[10] + m[14] + m[15]);
+    const C rhoN = (SHIFT ? C(1) + S : S) / (C(1) + v1);
+    C Nyx = (C(1.0) / 2) * (m[0] + m[8] + m[9] - m[1] - m[12] - m[13]) - (C(1.0) / 3) * (rhoN * v0);
+    C Nyz = (C(1.0) / 2) * (m[4] + m[8] + m[12] - m[5] - m[9] - m[13]) - (C(1.0) / 3) * (rhoN * v2);
+    m[3] = m[2] - (1 / C(3.0)) * rhoN * v1;
+    m[11] = m[6] + (1 / C(6.0)) * rhoN * (-v0 - v1) + Nyx;
+    m[7] = m[10] + (1 / C(6.0)) * rhoN * (-v1 + v0) - Nyx;
+    m[17] = m[14] + (1 / C(6.0)) * rhoN * (-v1 - v2) + Nyz;
+    m[16] = m[15] + (1 / C(6.0)) * rhoN * (-v1 + v2) - Nyz;
+  }
+  if (normal[1] < 0) {
+    v0 = C(0.0); v1 = C(1.0); v2 = C(0.0);
+    const C S = m[0] + m[1] + m[4] + m[5] + m[8] + m[9] + m[12] + m[13] + m[18] +
+              C(2) * (m[3] + m[7] + m[11] + m[16] + m[17]);
+    const C rhoN = (SHIFT ? C(1) + S : S) / (C(1) - v1);
+    C Nyx = (C(1.0) / 2) * (m[0] + m[8] + m[9] - m[1] - m[12] - m[13]) - (C(1.0) / 3) * (rhoN * v0);
+    C Nyz = (C(1.0) / 2) * (m[4] + m[8] + m[12] - m[5] - m[9] - m[13]) - (C(1.0) / 3) * (rhoN * v2);
+    m[2] = m[3] + (1 / C(3.0)) * rhoN * v1;
+    m[6] = m[11] + (1 / C(6.0)) * rhoN * (v0 + v1) - Nyx;
+    m[10] = m[7] + (1 / C(6.0)) * rhoN * (v1 - v0) + Nyx;
+    m[14] = m[17] + (1 / C(6.0)) * rhoN * (v1 + v2) - Nyz;
+    m[15] = m[16] + (1 / C(6.0)) * rhoN * (v1 - v2) + Nyz;
+  }
+  if (normal[2] > 0) {
+    v0 = C(0.0); v1 = C(0.0); v2 = vel;
+    const C S = m[0] + m[1] + m[4] + m[5] + m[8] + m[9] + m[12] + m[13] + m[18] +
+              C(2) * (m[3] + m[7] + m[11] + m[16] + m[17]);
+    const C rhoN = (SHIFT ? C(1) + S : S) / (C(1) + v2);
+    C Nzx = (C(1.0) / 2) * (m[0] + m[6] + m[7] - m[1] - m[10] - m[11]) - (C(1.0) / 3) * (rhoN * v0);
+    C Nzy = (C(1.0) / 2) * (m[2] + m[6] + m[10] - m[3] - m[7] - m[11]) - (C(1.0) / 3) * (rhoN * v1);
+    m[5] = m[4] - (1 / C(3.0)) * rhoN * v2;
+    m[9] = m[12] + (1 / C(6.0)) * rhoN * (v0 - v2) - Nzx;
+    m[13] = m[8] + (1 / C(6.0)) * rhoN * (-v0 - v2) + Nzx;
+    m[15] = m[16] + (1 / C(6.0)) * rhoN * (v1 - v2) - Nzy;
+    m[17] = m[14] + (1 / C(6.0)) * rhoN * (-v1 - v2) + Nzy;
+  }
+  if (normal[2] < 0) {
+    v0 = C(0.0); v1 = C(0.0); v2 = vel;
+    const C S = m[0] + m[1] + m[2] + m[3] + m[6] + m[7] + m[10] + m[11] + m[18] +
+              C(2) * (m[5] + m[9] + m[13] + m[15] + m[17]);
+    const C rhoN = (SHIFT ? C(1) + S : S) / (C(1) - v2);
+    C Nzx = (C(1.0) / 2) * (m[0] + m[6] + m[7] - m[1] - m[10] - m[11]) - (C(1.0) / 3) * (rhoN * v0);
+    C Nzy = (C(1.0) / 2) * (m[2] + m[6] + m[10] - m[3] - m[7] - m[11]) - (C(1.0) / 3) * (rhoN * v1);
+    m[4] = m[5] + (1 / C(3.0)) * rhoN * v2;
+    m[8] = m[13] + (1 / C(6.0)) * rhoN * (v0 + v2) - Nzx;
+    m[12] = m[9] + (1 / C(6.0)) * rhoN * (-v0 + v2) + Nzx;
+    m[14] = m[17] + (1 / C(6.0)) * rhoN * (v1 + v2) - Nzy;
+    m[16] = m[15] + (1 / C(6.0)) * rhoN * (-v1 + v2) + Nzy;
+  }
+}
+
+template <class L, class S>
+struct Kern {
+  using P = Pop<S>;
+  using C = typename P::C;
+  static constexpr int Q = L::Q;
+  static constexpr bool SHIFT = P::shifted;
+
+  // Post-stream populations of slot p (pull), before any boundary rule.
+  // Missing links take the slot's previous post-BC value (reference: the
+  // target of a push that never happens keeps its value, Simulation.cpp:245-251)
+  // or, in half-way bounce-back mode, the slot's own opposite population.
+  __device__ static void gather(const StepArgs& a, int32_t p, C* F, uint32_t& miss) {
+    const bool special = p < a.n_special_pad;
+    miss = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int32_t s = (q == L::REST) ? p : a.src[int64_t(q) * a.npad + p];
+      if (s >= 0) {
+        C v = P::ld(a.fs_in, int64_t(q) * a.npad + s);
+        if (a.zero_check && v == C(0)) {
+          // Simulation.cpp:247: a zero f* is not streamed.
+          if (special) v = C(a.fprev_in[int64_t(q) * a.n_special_pad + p]);
+          else atomicOr(a.err_flags, ERRF_ZERO_SKIP_BULK);
+        }
+        F[q] = v;
+      } else {
+        miss |= 1u << q;
+        if (a.wall_mode == WALL_BB) F[q] = P::ld(a.fs_in, int64_t(L::opp[q]) * a.npad + p);
+        else F[q] = C(a.fprev_in[int64_t(q) * a.n_special_pad + p]);
+      }
+    }
+  }
+
+  __device__ static C inlet_speed(const StepArgs& a, int32_t p) {
+    const int32_t i = a.inlet_slot[p];
+    double u = 0.0;
+    for (int m = 0; m < a.n_modes; ++m) u += a.coef[m] * a.inlet_basis[int64_t(m) * a.n_inlet + i];
+    return C(u);
+  }
+
+  __device__ static void equilibrium(C* e, C rho, C ux, C uy, C uz) {
+    const C usq = ux * ux + uy * uy + uz * uz;
+    const C drho = SHIFT ? rho - C(1) : C(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) e[q] = geq_of<L, C, SHIFT>(q, drho, rho, ux, uy, uz, usq);
+  }
+
+  __device__ static void density_velocity(const C* f, C& rho, C& ux, C& uy, C& uz) {
+    C s, jx, jy, jz;
+    moments<L, C, SHIFT>(f, s, jx, jy, jz);
+    rho = SHIFT ? C(1) + s : s;
+    ux = jx / rho;
+    uy = jy / rho;
+    uz = jz / rho;
+  }
+
+  // Non-equilibrium extrapolation from the interior neighbour's post-stream
+  // populations G: f = feq(rho_b, u_b) + (G - feq(rho_G, u_G)).
+  __device__ static void nee(C* F, const C* G, C rho_b, C ux, C uy, C uz, bool use_G_velocity) {
+    C rg, gx, gy, gz;
+    density_velocity(G, rg, gx, gy, gz);
+    if (use_G_velocity) {
+      ux = gx;
+      uy = gy;
+      uz = gz;
+    }
+    if (rho_b <= C(0)) rho_b = rg;
+    C eb[Q], eg[Q];
+    equilibrium(eb, rho_b, ux, uy, uz);
+    equilibrium(eg, rg, gx, gy, gz);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) F[q] = eb[q] + (G[q] - eg[q]);
+  }
+
+  // Simulation::ComputeWallShearStress for one node (Simulation.cpp:767-799),
+  // loops and accumulation order as in the reference; f_eq is rebuilt from the
+  // stored density/velocity of the previous collision.
+  __device__ static void wss(const StepArgs& a, int32_t p, const C* F) {
+    const int64_t sp = a.n_special_pad;
+    const double n[3] = {a.normal[p], a.normal[sp + p], a.normal[2 * sp + p]};
+    const double r0 = a.rho_u[p], u0 = a.rho_u[sp + p], u1 = a.rho_u[2 * sp + p], u2 = a.rho_u[3 * sp + p];
+    double fne[Q];
+    if (SHIFT) {
+      const C drho = C(r0), rho = C(1) + drho, ux = C(u0), uy = C(u1), uz = C(u2);
+      const C usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) fne[q] = double(F[q] - geq_of<L, C, true>(q, drho, rho, ux, uy, uz, usq));
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) fne[q] = double(F[q]) - feq_ref<L>(q, r0, u0, u1, u2);
+    }
+    double tau[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double sum_alpha = 0.0;
+#pragma unroll
+      for (int al = 0; al < Q; ++al) {
+        const int c[3] = {L::cx[al], L::cy[al], L::cz[al]};
+        double sum_k = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sum_k += c[k] * n[k] * n[i];
+        double sum_j = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sum_j += c[j] * n[j] * (c[i] - sum_k);
+        sum_alpha += fne[al] * sum_j;
+      }
+      tau[i] = a.wss_C * sum_alpha;
+    }
+    const double W = sqrt(tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]);
+    a.wss[p] = W;
+    a.wss[sp + p] += tau[0];
+    a.wss[2 * sp + p] += tau[1];
+    a.wss[3 * sp + p] += tau[2];
+    a.wss[4 * sp + p] += W;
+  }
+
+  template <int MODE, bool FORCE>
+  __device__ static void collide_store(const StepArgs& a, int32_t p, C* F, bool wall_zero_u, bool nearwall,
+                                       bool special) {
+    C drho, ux, uy, uz;
+    if constexpr (!SHIFT && !FORCE) {
+      double rho, dux, duy, duz;
+      collide_ref<L>(F, wall_zero_u, a.omega, rho, dux, duy, duz);
+      drho = C(rho);
+      ux = C(dux);
+      uy = C(duy);
+      uz = C(duz);
+    } else {
+      const C frc[3] = {C(a.force[0]), C(a.force[1]), C(a.force[2])};
+      const C alpha = a.porosity ? C(a.porosity[p]) : C(0);
+      collide_ext<L, C, SHIFT, FORCE>(F, C(a.omega), frc, alpha, wall_zero_u, drho, ux, uy, uz);
+    }
+    if (special && nearwall) {
+      const int64_t sp = a.n_special_pad;
+      a.rho_u[p] = double(drho);
+      a.rho_u[sp + p] = double(ux);
+      a.rho_u[2 * sp + p] = double(uy);
+      a.rho_u[3 * sp + p] = double(uz);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) P::st(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+  }
+
+  __device__ static void load_direct(const StepArgs& a, int32_t p, C* F) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double f = a.f_direct ? a.f_direct[int64_t(q) * a.npad + p] : L::w[q];
+      F[q] = SHIFT ? C(f - L::w[q]) : C(f);
+    }
+  }
+
+  __device__ static void write_direct(const StepArgs& a, int32_t p, const C* F) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a.f_out[int64_t(q) * a.npad + p] = SHIFT ? double(F[q]) + L::w[q] : double(F[q]);
+  }
+
+  template <int MODE, bool FORCE>
+  __device__ static void special_node(const StepArgs& a, int32_t p) {
+    const uint32_t fl = a.sflags[p];
+    if (fl & SFLAG_PAD) return;
+    const int kind = int(fl & SFLAG_KIND_MASK);
+    const bool nearwall = (fl & SFLAG_NEARWALL) != 0;
+    C F[Q];
+    if (MODE == KMODE_INIT) {
+      load_direct(a, p, F);
+    } else {
+      uint32_t miss;
+      gather(a, p, F, miss);
+      if (kind == 2 /*inlet*/) {
+        if (a.inlet_mode == INLET_ZOUHE) {
+          if constexpr (Q == 19) zouhe_ref<C, SHIFT>(F, inlet_speed(a, p), a.inlet_normal);
+        } else if (a.inlet_mode == INLET_NEE) {
+          const int32_t u = a.bc_nbr[p];
+          if (u >= 0) {
+            C G[Q];
+            uint32_t gm;
+            gather(a, u, G, gm);
+            const C U = inlet_speed(a, p);
+            nee(F, G, C(-1), -C(a.inlet_normal[0]) * U, -C(a.inlet_normal[1]) * U, -C(a.inlet_normal[2]) * U,
+                false);
+          }
+        }
+      } else if (kind == 3 /*outlet*/) {
+        const int32_t u = a.bc_nbr[p];
+        if (u >= 0) {
+          if (a.outlet_mode == OUTLET_REF) {
+            // Simulation::ApplyOutletBc: copy every population of the node
+            // behind the outlet after its own stream and inlet rule.
+            C G[Q];
+            uint32_t gm;
+            gather(a, u, G, gm);
+            if (u < a.n_special_pad && int(a.sflags[u] & SFLAG_KIND_MASK) == 2 && a.inlet_mode == INLET_ZOUHE) {
+              if constexpr (Q == 19) zouhe_ref<C, SHIFT>(G, inlet_speed(a, u), a.inlet_normal);
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) F[q] = G[q];
+          } else if (a.outlet_mode == OUTLET_NEE) {
+            C G[Q];
+            uint32_t gm;
+            gather(a, u, G, gm);
+            nee(F, G, C(a.outlet_rho), C(0), C(0), C(0), true);
+          }
+        }
+      } else if (kind == 1 /*wall*/ && a.wall_mode == WALL_REF) {
+        // Simulation::ApplyWallBc: for q in order, if the push neighbour
+        // x + c_q is absent (pull source of opp(q) missing), f[opp] = f[q].
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+          if (miss & (1u << L::opp[q])) F[L::opp[q]] = F[q];
+      }
+      if (MODE == KMODE_STEP && nearwall && a.wss_on) wss(a, p, F);
+    }
+    if (MODE == KMODE_MATERIALIZE) {
+      write_direct(a, p, F);
+      return;
+    }
+    // Reference post-BC f of boundary nodes, for links that stream nothing.
+    if (a.fprev_out) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) a.fprev_out[int64_t(q) * a.n_special_pad + p] = double(F[q]);
+    }
+    const bool zero_u = (kind == 1) && (a.wall_mode == WALL_REF);
+    collide_store<MODE, FORCE>(a, p, F, zero_u, nearwall, true);
+  }
+
+  template <int MODE, bool FORCE>
+  __device__ static void bulk_node(const StepArgs& a, int32_t p) {
+    C F[Q];
+    if (MODE == KMODE_INIT) {
+      load_direct(a, p, F);
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int32_t s = (q == L::REST) ? p : a.src[int64_t(q) * a.npad + p];
+        const C v = P::ld(a.fs_in, int64_t(q) * a.npad + s);
+        if (!SHIFT && a.zero_check && v == C(0)) atomicOr(a.err_flags, ERRF_ZERO_SKIP_BULK);
+        F[q] = v;
+      }
+    }
+    if (MODE == KMODE_MATERIALIZE) {
+      write_direct(a, p, F);
+      return;
+    }
+    collide_store<MODE, FORCE>(a, p, F, false, false, false);
+  }
+};
+
+__device__ inline int32_t bulk_chunk(const StepArgs& a, int32_t b) {
+  if (!a.xcd_remap) return b - a.blocks_special;
+  const int x = b & 7;
+  return a.xcd_offset[x] + (b - a.xcd_first[x]) / 8;
+}
+
+template <class L, class S, int MODE, bool FORCE>
+__global__ __launch_bounds__(BLOCK) void lbm_kernel(StepArgs a) {
+  const int32_t b = blockIdx.x;
+  if (b < a.blocks_special) {
+    const int32_t p = b * BLOCK + threadIdx.x;
+    if (p < a.n_special_pad) Kern<L, S>::template special_node<MODE, FORCE>(a, p);
+    return;
+  }
+  const int32_t p = a.n_special_pad + bulk_chunk(a, b) * BLOCK + threadIdx.x;
+  if (p < a.bulk_end) Kern<L, S>::template bulk_node<MODE, FORCE>(a, p);
+}
+
+// ------------------------------------------------------------- reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-level sum of NV doubles; thread 0 writes partials[blockIdx * NV + i].
+template <int NV>
+__device__ inline void block_sum_store(double* v, double* partials) {
+  __shared__ double red[BLOCK / 64][NV];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[wv][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+      for (int w = 0; w < BLOCK / 64; ++w) s += red[w][i];
+      partials[int64_t(blockIdx.x) * NV + i] = s;
+    }
+}
+
+template <class L, class S>
+__global__ __launch_bounds__(BLOCK) void moments_kernel(StepArgs a, double* partials) {
+  using P = Pop<S>;
+  const int32_t p = blockIdx.x * BLOCK + threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  const bool valid = (p < a.n_special_pad) ? (p < a.n_special && !(a.sflags[p] & SFLAG_PAD))
+                                           : (p < a.bulk_end);
+  if (valid) {
+    double s = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+      double f = double(P::ld(a.fs_in, int64_t(q) * a.npad + p));
+      if (P::shifted) f += L::w[q];
+      s += f;
+      jx += L::cx[q] * f;
+      jy += L::cy[q] * f;
+      jz += L::cz[q] * f;
+    }
+    v[0] = s;
+    v[1] = jx;
+    v[2] = jy;
+    v[3] = jz;
+  }
+  block_sum_store<4>(v, partials);
+}
+
+__global__ __launch_bounds__(BLOCK) void wss_stats_kernel(StepArgs a, double* partials) {
+  const int32_t p = blockIdx.x * BLOCK + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (p < a.n_special && (a.sflags[p] & SFLAG_NEARWALL) && !(a.sflags[p] & SFLAG_PAD)) {
+    const double w = a.wss[p];
+    v[0] = w * w;
+    v[1] = w;
+  }
+  __shared__ double red[BLOCK / 64][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  v[0] = wave_sum(v[0]);
+  v[1] = wave_max(v[1]);
+  if (lane == 0) {
+    red[wv][0] = v[0];
+    red[wv][1] = v[1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0, m = 0.0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+      s += red[w][0];
+      m = fmax(m, red[w][1]);
+    }
+    partials[2 * blockIdx.x] = s;
+    partials[2 * blockIdx.x + 1] = m;
+  }
+}
+
+// Fixed-order final pass (deterministic): one block folds nblk partials.
+template <int NV, bool MAXLAST>
+__global__ __launch_bounds__(BLOCK) void finish_kernel(const double* partials, int32_t nblk, double* out) {
+  __shared__ double red[BLOCK][NV];
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+  for (int32_t b = threadIdx.x; b < nblk; b += BLOCK)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const double x = partials[int64_t(b) * NV + i];
+      acc[i] = (MAXLAST && i == NV - 1) ? fmax(acc[i], x) : acc[i] + x;
+    }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) red[threadIdx.x][i] = acc[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+      for (int t = 0; t < BLOCK; ++t) s = (MAXLAST && i == NV - 1) ? fmax(s, red[t][i]) : s + red[t][i];
+      out[i] = s;
+    }
+  }
+}
+
+template <class L, class S, int MODE, bool FORCE>
+hipError_t launch_one(const StepArgs& a, hipStream_t st) {
+  const dim3 grid(a.blocks_special + a.blocks_bulk);
+  if (grid.x == 0) return hipSuccess;
+  hipLaunchKernelGGL((lbm_kernel<L, S, MODE, FORCE>), grid, dim3(BLOCK), 0, st, a);
+  return hipGetLastError();
+}
+
+template <class L, class S>
+hipError_t launch_mode(int kmode, bool force, const StepArgs& a, hipStream_t st) {
+  switch (kmode) {
+    case KMODE_STEP:
+      return force ? launch_one<L, S, KMODE_STEP, true>(a, st) : launch_one<L, S, KMODE_STEP, false>(a, st);
+    case KMODE_INIT:
+      return force ? launch_one<L, S, KMODE_INIT, true>(a, st) : launch_one<L, S, KMODE_INIT, false>(a, st);
+    case KMODE_MATERIALIZE:
+      return launch_one<L, S, KMODE_MATERIALIZE, false>(a, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class L>
+hipError_t launch_prec(int precision, int kmode, bool force, const StepArgs& a, hipStream_t st) {
+  switch (precision) {
+    case 0: return launch_mode<L, double>(kmode, force, a, st);
+    case 1: return launch_mode<L, float>(kmode, force, a, st);
+    case 2: return launch_mode<L, __half>(kmode, force, a, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class L, class S>
+hipError_t moments_one(const StepArgs& a, double* partials, double* out4, hipStream_t st) {
+  const int32_t nblk = static_cast<int32_t>((a.bulk_end + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL((moments_kernel<L, S>), dim3(nblk), dim3(BLOCK), 0, st, a, partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((finish_kernel<4, false>), dim3(1), dim3(BLOCK), 0, st, partials, nblk, out4);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_lbm(int Q, int precision, int kmode, bool force, const StepArgs& a, hipStream_t st) {
+  if (Q == 19) return launch_prec<D3Q19>(precision, kmode, force, a, st);
+  if (Q == 27) return launch_prec<D3Q27>(precision, kmode, force, a, st);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_moments(int Q, int precision, const StepArgs& a, double* partials, double* out4,
+                          hipStream_t st) {
+  if (Q == 19) {
+    if (precision == 0) return moments_one<D3Q19, double>(a, partials, out4, st);
+    if (precision == 1) return moments_one<D3Q19, float>(a, partials, out4, st);
+    if (precision == 2) return moments_one<D3Q19, __half>(a, partials, out4, st);
+  } else if (Q == 27) {
+    if (precision == 0) return moments_one<D3Q27, double>(a, partials, out4, st);
+    if (precision == 1) return moments_one<D3Q27, float>(a, partials, out4, st);
+    if (precision == 2) return moments_one<D3Q27, __half>(a, partials, out4, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_wss_stats(const StepArgs& a, double* partials, double* out2, hipStream_t st) {
+  const int32_t nblk = (a.n_special_pad + BLOCK - 1) / BLOCK;
+  if (nblk == 0) return hipMemsetAsync(out2, 0, 2 * sizeof(double), st);
+  hipLaunchKernelGGL(wss_stats_kernel, dim3(nblk), dim3(BLOCK), 0, st, a, partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((finish_kernel<2, true>), dim3(1), dim3(BLOCK), 0, st, partials, nblk, out2);
+  return hipGetLastError();
+}
+
+}  // namespace lbm

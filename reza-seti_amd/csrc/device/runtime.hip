@@ -1,0 +1,670 @@
+// Device half of the C ABI: solver handle, slot layout, launches.
+//
+// Replaces the state and time loop of Simulation (Simulation.cpp:44-58,
+// 538-618) for a single rank. The MPI halo exchange (Simulation::Communicate,
+// Simulation.cpp:270-516) has nothing to do on one GPU: every node lives in
+// one slot list and streaming reads neighbours directly.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../capi_internal.h"
+#include "../lattice.h"
+#include "kernels.h"
+
+using namespace lbm;
+
+namespace {
+
+// Pulsatile::InletVelequation1..3 (pulsatile.h:55-69).
+double pulse1(double x) {
+  return (-8.579e+08 * pow(x, 8) + 4.059e+08 * pow(x, 7) - 7.817e+07 * pow(x, 6) + 7.994e+06 * pow(x, 5) -
+          4.835e+05 * pow(x, 4) + 1.782e+04 * pow(x, 3) - 321.6 * pow(x, 2) + 3.215 * x + 0.1423);
+}
+double pulse2(double x) {
+  return (-4.32e+06 * pow(x, 8) + 7.94e+06 * pow(x, 7) - 6.298e+06 * pow(x, 6) + 2.82e+06 * pow(x, 5) -
+          7.809e+05 * pow(x, 4) + 1.37e+05 * pow(x, 3) - 1.484e+04 * pow(x, 2) + 901.9 * x - 22.87);
+}
+double pulse3(double x) {
+  return (-5.123 * pow(x, 4) + 14.15 * pow(x, 3) - 14.21 * pow(x, 2) + 5.865 * x - 0.5696);
+}
+
+// Simulation::getInletVelocity (Simulation.cpp:519-534).
+double inlet_velocity(double x, double period) {
+  double vel = 0.0;
+  const double nx = fmod(x, period);
+  if (nx >= 0 && nx < (0.1 * period)) vel = pulse1(nx);
+  else if (nx >= (0.1 * period) && nx < (0.3 * period)) vel = pulse2(nx);
+  else if (nx >= (0.3 * period) && nx < (0.9 * period)) vel = pulse3(nx);
+  return vel;
+}
+
+float half_to_float(uint16_t h) {
+  const uint32_t s = (h >> 15) & 1u, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  float v;
+  if (e == 0) v = std::ldexp(float(m), -24);
+  else if (e == 31) v = m ? std::numeric_limits<float>::quiet_NaN() : std::numeric_limits<float>::infinity();
+  else v = std::ldexp(float(m | 0x400u), int(e) - 25);
+  return s ? -v : v;
+}
+
+#define HIP_TRY(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return fail(LBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));               \
+  } while (0)
+
+}  // namespace
+
+struct lbm_solver {
+  lbm_params prm{};
+  int Q = 19;
+  int prec = 0;
+  size_t esize = 8;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  int64_t N = 0, n_special = 0, n_special_pad = 0, n_bulk = 0, bulk_end = 0, npad = 0;
+  std::vector<int32_t> slot_of;   // reference node -> slot
+  std::vector<int32_t> node_of;   // slot -> reference node, or -1
+  std::vector<int32_t> nw_slots;  // near-wall list order -> slot
+  std::vector<int32_t> inlet_nodes;
+  std::vector<uint8_t> kind;      // per reference node
+
+  void* fs[2] = {nullptr, nullptr};
+  double* fprev[2] = {nullptr, nullptr};
+  int32_t* src = nullptr;
+  uint32_t* sflags = nullptr;
+  int32_t* bc_nbr = nullptr;
+  int32_t* inlet_slot = nullptr;
+  double* inlet_basis = nullptr;
+  double* normal = nullptr;
+  double* rho_u = nullptr;
+  double* wss = nullptr;
+  float* porosity = nullptr;
+  uint32_t* err = nullptr;
+  double* f_tmp = nullptr;
+  double* partials = nullptr;
+  double* red_out = nullptr;
+  size_t device_bytes = 0;
+
+  int cur = 0;
+  int64_t iter = 0;
+  double total_runtime = 0.0;
+  bool stepped = false;              // a step ran since the last set/create
+  std::vector<double> last_set;      // f given to the last set (node-major), empty = w
+  int n_modes = 1;
+  double last_coef[MAX_MODES] = {0};
+  StepArgs base{};
+};
+
+namespace {
+
+template <class T>
+int dalloc(lbm_solver* s, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return LBM_OK;
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+  s->device_bytes += count * sizeof(T);
+  return LBM_OK;
+}
+
+int dalloc_raw(lbm_solver* s, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return LBM_OK;
+  HIP_TRY(hipMalloc(p, bytes));
+  s->device_bytes += bytes;
+  return LBM_OK;
+}
+
+void free_all(lbm_solver* s) {
+  void* ptrs[] = {s->fs[0], s->fs[1], s->fprev[0], s->fprev[1], s->src, s->sflags, s->bc_nbr, s->inlet_slot,
+                  s->inlet_basis, s->normal, s->rho_u, s->wss, s->porosity, s->err, s->f_tmp, s->partials,
+                  s->red_out};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+StepArgs args_for(lbm_solver* s, int in) {
+  StepArgs a = s->base;
+  a.fs_in = s->fs[in];
+  a.fs_out = s->fs[1 - in];
+  a.fprev_in = s->fprev[in];
+  a.fprev_out = s->fprev[1 - in];
+  return a;
+}
+
+int ensure_tmp(lbm_solver* s) {
+  if (s->f_tmp) return LBM_OK;
+  return dalloc(s, &s->f_tmp, static_cast<size_t>(s->Q) * s->npad);
+}
+
+// Collide a given f (null = f = w_q everywhere) into the current state.
+int init_state(lbm_solver* s, const double* f_host) {
+  StepArgs a = args_for(s, 1 - s->cur);  // writes fs[cur], fprev[cur]
+  a.f_direct = nullptr;
+  if (f_host) {
+    int rc = ensure_tmp(s);
+    if (rc) return rc;
+    std::vector<double> soa(static_cast<size_t>(s->Q) * s->npad, 0.0);
+    const lbm::LatticeView L = lbm::lattice_view(s->Q);
+    for (int64_t slot = 0; slot < s->npad; ++slot) {
+      const int32_t n = s->node_of[slot];
+      for (int q = 0; q < s->Q; ++q)
+        soa[static_cast<size_t>(q) * s->npad + slot] = n >= 0 ? f_host[int64_t(n) * s->Q + q] : L.w[q];
+    }
+    HIP_TRY(hipMemcpyAsync(s->f_tmp, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    a.f_direct = s->f_tmp;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+  }
+  a.wss_on = 0;
+  HIP_TRY(launch_lbm(s->Q, s->prec, KMODE_INIT, s->porosity || s->prm.force[0] != 0.0 ||
+                                                    s->prm.force[1] != 0.0 || s->prm.force[2] != 0.0,
+                     a, s->stream));
+  s->stepped = false;
+  if (f_host) s->last_set.assign(f_host, f_host + static_cast<size_t>(s->N) * s->Q);
+  else s->last_set.clear();
+  return LBM_OK;
+}
+
+bool has_force(const lbm_solver* s) {
+  return s->porosity != nullptr || s->prm.force[0] != 0.0 || s->prm.force[1] != 0.0 || s->prm.force[2] != 0.0;
+}
+
+int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
+  if (nsteps < 0) return fail(LBM_ERR_ARG, "nsteps < 0");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  const bool force = has_force(s);
+  for (int32_t t = 0; t < nsteps; ++t) {
+    StepArgs a = args_for(s, s->cur);
+    if (coef) {
+      for (int m = 0; m < s->n_modes; ++m) a.coef[m] = coef[int64_t(t) * s->n_modes + m];
+    } else {
+      const double U = (s->prm.flow_type == LBM_FLOW_PULSATILE) ? inlet_velocity(s->total_runtime, 0.9)
+                                                                 : s->prm.cte_vel;
+      for (int m = 0; m < MAX_MODES; ++m) a.coef[m] = 0.0;
+      a.coef[0] = U;
+    }
+    a.wss_on = (s->prm.wss_every > 0 && (s->iter % s->prm.wss_every) == 0) ? 1 : 0;
+    HIP_TRY(launch_lbm(s->Q, s->prec, KMODE_STEP, force, a, s->stream));
+    std::memcpy(s->last_coef, a.coef, sizeof(s->last_coef));
+    s->cur = 1 - s->cur;
+    s->iter++;
+    if (s->prm.flow_type == LBM_FLOW_PULSATILE) s->total_runtime += s->prm.dt;
+    s->stepped = true;
+  }
+  if (s->base.zero_check && nsteps > 0) {
+    uint32_t flags = 0;
+    HIP_TRY(hipMemcpyAsync(&flags, s->err, sizeof(flags), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (flags) {
+      HIP_TRY(hipMemsetAsync(s->err, 0, sizeof(uint32_t), s->stream));
+      return fail(LBM_ERR_PARITY,
+                  "an f* == 0.0 population was not streamed into a plain fluid node; exact reference "
+                  "replay would need its previous populations");
+    }
+  }
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void lbm_params_default(lbm_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->precision = LBM_PREC_F64;
+  p->wall_mode = LBM_WALL_REFERENCE;
+  p->inlet_mode = LBM_INLET_REFERENCE_ZOUHE;
+  p->outlet_mode = LBM_OUTLET_REFERENCE;
+  p->flow_type = LBM_FLOW_CONSTANT;
+  p->wss_every = 1;
+  p->parity_checks = 1;
+  p->tau = 1.0;
+  p->dt = 1.0;
+  p->outlet_rho = 1.0;
+}
+
+int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
+  if (!g || !p || !out) return fail(LBM_ERR_ARG, "null argument");
+  *out = nullptr;
+  const Tables& t = g->t;
+  if (p->precision < 0 || p->precision > 2) return fail(LBM_ERR_ARG, "unknown precision");
+  if (!(p->tau > 0.5)) return fail(LBM_ERR_ARG, "tau must exceed 0.5");
+  if (p->wall_mode < 0 || p->wall_mode > 1 || p->inlet_mode < 0 || p->inlet_mode > 2 || p->outlet_mode < 0 ||
+      p->outlet_mode > 2)
+    return fail(LBM_ERR_ARG, "unknown boundary mode");
+  if (p->inlet_mode == LBM_INLET_REFERENCE_ZOUHE && !t.inlet_nodes.empty()) {
+    if (t.Q != 19)
+      return fail(LBM_ERR_UNSUPPORTED,
+                  "the reference Zou-He inlet (Boundary.cpp:111-289) is written for D3Q19 only");
+    if (!t.has_inlet_normal) return fail(LBM_ERR_ARG, "inlet nodes without an inlet normal");
+  }
+  if (p->outlet_mode == LBM_OUTLET_REFERENCE)
+    for (const auto& lst : t.outlet_nodes)
+      for (int32_t n : lst) {
+        const int32_t u = t.bc_nbr[n];
+        if (u >= 0 && t.kind[u] == KIND_OUTLET)
+          return fail(LBM_ERR_UNSUPPORTED,
+                      "an outlet node copies from another outlet node; the reference result then depends "
+                      "on its sequential processing order");
+      }
+
+  auto* s = new lbm_solver();
+  s->prm = *p;
+  s->Q = t.Q;
+  s->prec = p->precision;
+  s->esize = p->precision == 0 ? 8 : (p->precision == 1 ? 4 : 2);
+  s->N = t.N;
+  s->kind = t.kind;
+  s->inlet_nodes = t.inlet_nodes;
+
+  auto bail = [&](int rc) {
+    free_all(s);
+    delete s;
+    return rc;
+  };
+  if (hipSetDevice(p->device) != hipSuccess) return bail(fail(LBM_ERR_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+    return bail(fail(LBM_ERR_HIP, "stream/event creation failed"));
+
+  // Slot layout: special nodes (wall, inlet, outlet, near-wall) first, then
+  // plain fluid, each in reference order.
+  const int64_t N = t.N;
+  std::vector<uint8_t> nearwall(N, 0);
+  for (int32_t n : t.nearwall) nearwall[n] = 1;
+  s->slot_of.assign(N, -1);
+  int64_t ns = 0;
+  for (int64_t n = 0; n < N; ++n)
+    if (t.kind[n] != KIND_FLUID || nearwall[n]) s->slot_of[n] = static_cast<int32_t>(ns++);
+  s->n_special = ns;
+  s->n_special_pad = round_up(ns, BLOCK);
+  int64_t nb = 0;
+  for (int64_t n = 0; n < N; ++n)
+    if (s->slot_of[n] < 0) s->slot_of[n] = static_cast<int32_t>(s->n_special_pad + nb++);
+  s->n_bulk = nb;
+  s->bulk_end = s->n_special_pad + nb;
+  s->npad = std::max<int64_t>(round_up(s->bulk_end, BLOCK), BLOCK);
+  if (s->npad >= (int64_t(1) << 31)) return bail(fail(LBM_ERR_ARG, "too many nodes for int32 slots"));
+  s->node_of.assign(s->npad, -1);
+  for (int64_t n = 0; n < N; ++n) s->node_of[s->slot_of[n]] = static_cast<int32_t>(n);
+  s->nw_slots.resize(t.nearwall.size());
+  for (size_t i = 0; i < t.nearwall.size(); ++i) s->nw_slots[i] = s->slot_of[t.nearwall[i]];
+
+  const int Q = t.Q;
+  const lbm::LatticeView L = lbm::lattice_view(Q);
+  const int64_t npad = s->npad, sp = s->n_special_pad;
+
+  // Pull table in slot space: source of direction q is the push neighbour in
+  // direction opp(q).
+  std::vector<int32_t> src(static_cast<size_t>(Q) * npad);
+  for (int64_t slot = 0; slot < npad; ++slot) {
+    const int32_t n = s->node_of[slot];
+    for (int q = 0; q < Q; ++q) {
+      int32_t v = static_cast<int32_t>(slot);
+      if (n >= 0 && q != L.rest) {
+        const int32_t m = t.nb[static_cast<size_t>(L.opp[q]) * N + n];
+        v = m >= 0 ? s->slot_of[m] : -1;
+      }
+      src[static_cast<size_t>(q) * npad + slot] = v;
+    }
+  }
+  std::vector<uint32_t> sflags(sp, SFLAG_PAD);
+  std::vector<int32_t> bcn(sp, -1), islot(sp, -1);
+  std::vector<double> nrm(3 * sp, 0.0);
+  for (int64_t slot = 0; slot < s->n_special; ++slot) {
+    const int32_t n = s->node_of[slot];
+    sflags[slot] = uint32_t(t.kind[n]) | (nearwall[n] ? SFLAG_NEARWALL : 0u);
+    if (t.bc_nbr[n] >= 0) bcn[slot] = s->slot_of[t.bc_nbr[n]];
+  }
+  for (size_t i = 0; i < t.inlet_nodes.size(); ++i) islot[s->slot_of[t.inlet_nodes[i]]] = static_cast<int32_t>(i);
+  for (size_t i = 0; i < t.nearwall.size(); ++i)
+    for (int a = 0; a < 3; ++a) nrm[a * sp + s->slot_of[t.nearwall[i]]] = t.normals[3 * i + a];
+
+  // Device buffers.
+  int rc = LBM_OK;
+  const size_t pop_bytes = static_cast<size_t>(Q) * npad * s->esize;
+  if ((rc = dalloc_raw(s, &s->fs[0], pop_bytes)) || (rc = dalloc_raw(s, &s->fs[1], pop_bytes)) ||
+      (rc = dalloc(s, &s->src, src.size())) || (rc = dalloc(s, &s->sflags, sp)) || (rc = dalloc(s, &s->bc_nbr, sp)) ||
+      (rc = dalloc(s, &s->inlet_slot, sp)) || (rc = dalloc(s, &s->normal, 3 * sp)) ||
+      (rc = dalloc(s, &s->rho_u, 4 * sp)) || (rc = dalloc(s, &s->wss, 5 * sp)) || (rc = dalloc(s, &s->err, 1)))
+    return bail(rc);
+  if (p->wall_mode == LBM_WALL_REFERENCE || p->parity_checks) {
+    if ((rc = dalloc(s, &s->fprev[0], static_cast<size_t>(Q) * sp)) ||
+        (rc = dalloc(s, &s->fprev[1], static_cast<size_t>(Q) * sp)))
+      return bail(rc);
+  }
+  const int64_t nblk = npad / BLOCK;
+  if ((rc = dalloc(s, &s->partials, static_cast<size_t>(4 * nblk + 2 * (sp / BLOCK) + 8))) ||
+      (rc = dalloc(s, &s->red_out, 8)))
+    return bail(rc);
+  const size_t ninl = std::max<size_t>(t.inlet_nodes.size(), 1);
+  if ((rc = dalloc(s, &s->inlet_basis, MAX_MODES * ninl))) return bail(rc);
+  std::vector<double> basis(MAX_MODES * ninl, 0.0);
+  for (size_t i = 0; i < ninl; ++i) basis[i] = 1.0;
+
+  auto up = [&](void* d, const void* h, size_t bytes) {
+    return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s->stream) : hipSuccess;
+  };
+  if (up(s->src, src.data(), src.size() * 4) || up(s->sflags, sflags.data(), sp * 4) ||
+      up(s->bc_nbr, bcn.data(), sp * 4) || up(s->inlet_slot, islot.data(), sp * 4) ||
+      up(s->normal, nrm.data(), nrm.size() * 8) || up(s->inlet_basis, basis.data(), basis.size() * 8) ||
+      hipMemsetAsync(s->rho_u, 0, 4 * sp * 8, s->stream) || hipMemsetAsync(s->wss, 0, 5 * sp * 8, s->stream) ||
+      hipMemsetAsync(s->err, 0, 4, s->stream) || hipMemsetAsync(s->fs[0], 0, pop_bytes, s->stream) ||
+      hipMemsetAsync(s->fs[1], 0, pop_bytes, s->stream))
+    return bail(fail(LBM_ERR_HIP, "upload failed"));
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(LBM_ERR_HIP, "upload sync failed"));
+
+  StepArgs& a = s->base;
+  std::memset(&a, 0, sizeof(a));
+  a.src = s->src;
+  a.npad = npad;
+  a.n_special = static_cast<int32_t>(s->n_special);
+  a.n_special_pad = static_cast<int32_t>(sp);
+  a.bulk_end = static_cast<int32_t>(s->bulk_end);
+  a.blocks_special = static_cast<int32_t>(sp / BLOCK);
+  a.blocks_bulk = static_cast<int32_t>((s->n_bulk + BLOCK - 1) / BLOCK);
+  a.xcd_remap = 1;
+  {
+    const int32_t Bs = a.blocks_special, Bb = a.blocks_bulk;
+    int32_t off = 0;
+    for (int x = 0; x < 8; ++x) {
+      const int32_t first = Bs + ((x - Bs % 8) + 8) % 8;
+      const int32_t cnt = first < Bs + Bb ? (Bs + Bb - 1 - first) / 8 + 1 : 0;
+      a.xcd_first[x] = first;
+      a.xcd_offset[x] = off;
+      off += cnt;
+    }
+  }
+  a.sflags = s->sflags;
+  a.bc_nbr = s->bc_nbr;
+  a.inlet_slot = s->inlet_slot;
+  a.n_inlet = static_cast<int32_t>(ninl);
+  a.n_modes = 1;
+  a.inlet_basis = s->inlet_basis;
+  for (int i = 0; i < 3; ++i) a.inlet_normal[i] = t.inlet_normal[i];
+  a.normal = s->normal;
+  a.rho_u = s->rho_u;
+  a.wss = s->wss;
+  // Simulation::ComputeWallShearStress constant (Simulation.cpp:768-771);
+  // `spacing` is the doubled lattice pitch (ComplexGeometry.cpp:157).
+  {
+    const double phys_den = 1060;
+    const double dtdt = p->dt * p->dt;
+    const double spacing = p->spacing > 0 ? p->spacing : t.grid.spacing;
+    const double dxdx = spacing * spacing * 1e-6;
+    a.wss_C = phys_den * dtdt / dxdx;
+  }
+  a.omega = 1.0 / p->tau;
+  for (int i = 0; i < 3; ++i) a.force[i] = p->force[i];
+  a.porosity = nullptr;
+  a.wall_mode = p->wall_mode;
+  a.inlet_mode = p->inlet_mode;
+  a.outlet_mode = p->outlet_mode;
+  a.outlet_rho = p->outlet_rho;
+  a.zero_check = (p->parity_checks && p->precision == LBM_PREC_F64 && p->wall_mode == LBM_WALL_REFERENCE) ? 1 : 0;
+  a.err_flags = s->err;
+
+  if ((rc = init_state(s, nullptr))) return bail(rc);
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(LBM_ERR_HIP, "init failed"));
+  *out = s;
+  return LBM_OK;
+}
+
+int lbm_set_inlet_profile(lbm_solver* s, int32_t n_modes, const double* basis) {
+  if (!s || n_modes < 1 || n_modes > MAX_MODES || !basis) return fail(LBM_ERR_ARG, "invalid argument");
+  const size_t ninl = s->inlet_nodes.size();
+  if (ninl == 0) return fail(LBM_ERR_ARG, "no inlet nodes");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  HIP_TRY(hipMemcpyAsync(s->inlet_basis, basis, n_modes * ninl * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->n_modes = n_modes;
+  s->base.n_modes = n_modes;
+  return LBM_OK;
+}
+
+int lbm_set_porosity(lbm_solver* s, const float* alpha) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  if (!alpha) {
+    if (s->porosity) (void)hipFree(s->porosity);
+    s->porosity = nullptr;
+    s->base.porosity = nullptr;
+    return LBM_OK;
+  }
+  if (!s->porosity) {
+    int rc = dalloc(s, &s->porosity, s->npad);
+    if (rc) return rc;
+  }
+  std::vector<float> h(s->npad, 0.0f);
+  for (int64_t slot = 0; slot < s->npad; ++slot)
+    if (s->node_of[slot] >= 0) h[slot] = alpha[s->node_of[slot]];
+  HIP_TRY(hipMemcpyAsync(s->porosity, h.data(), h.size() * 4, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->base.porosity = s->porosity;
+  return LBM_OK;
+}
+
+int lbm_step(lbm_solver* s, int32_t nsteps) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  return run_steps(s, nsteps, nullptr);
+}
+
+int lbm_step_coefficients(lbm_solver* s, int32_t nsteps, const double* coef) {
+  if (!s || !coef) return fail(LBM_ERR_ARG, "null argument");
+  return run_steps(s, nsteps, coef);
+}
+
+int lbm_step_timed(lbm_solver* s, int32_t nsteps, double* ms) {
+  if (!s || !ms) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  HIP_TRY(hipEventRecord(s->ev0, s->stream));
+  int rc = run_steps(s, nsteps, nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(s->ev1, s->stream));
+  HIP_TRY(hipEventSynchronize(s->ev1));
+  float e = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&e, s->ev0, s->ev1));
+  *ms = e;
+  return LBM_OK;
+}
+
+int lbm_set_populations(lbm_solver* s, const double* f) {
+  if (!s || !f) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  int rc = init_state(s, f);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_get_populations(lbm_solver* s, double* f) {
+  if (!s || !f) return fail(LBM_ERR_ARG, "null argument");
+  const int Q = s->Q;
+  if (!s->stepped) {
+    if (!s->last_set.empty()) {
+      std::memcpy(f, s->last_set.data(), s->last_set.size() * sizeof(double));
+    } else {
+      const lbm::LatticeView L = lbm::lattice_view(Q);
+      for (int64_t n = 0; n < s->N; ++n)
+        for (int q = 0; q < Q; ++q) f[n * Q + q] = L.w[q];
+    }
+    return LBM_OK;
+  }
+  HIP_TRY(hipSetDevice(s->prm.device));
+  int rc = ensure_tmp(s);
+  if (rc) return rc;
+  // Re-run the last step's stream + boundary stage into f_tmp.
+  StepArgs a = args_for(s, 1 - s->cur);
+  std::memcpy(a.coef, s->last_coef, sizeof(a.coef));
+  a.f_out = s->f_tmp;
+  a.wss_on = 0;
+  HIP_TRY(launch_lbm(Q, s->prec, KMODE_MATERIALIZE, false, a, s->stream));
+  std::vector<double> soa(static_cast<size_t>(Q) * s->npad);
+  HIP_TRY(hipMemcpyAsync(soa.data(), s->f_tmp, soa.size() * 8, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int64_t n = 0; n < s->N; ++n) {
+    const int64_t slot = s->slot_of[n];
+    for (int q = 0; q < Q; ++q) f[n * Q + q] = soa[static_cast<size_t>(q) * s->npad + slot];
+  }
+  return LBM_OK;
+}
+
+int lbm_get_post_collision(lbm_solver* s, double* fs) {
+  if (!s || !fs) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  const int Q = s->Q;
+  const size_t cnt = static_cast<size_t>(Q) * s->npad;
+  std::vector<unsigned char> raw(cnt * s->esize);
+  HIP_TRY(hipMemcpyAsync(raw.data(), s->fs[s->cur], raw.size(), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const lbm::LatticeView L = lbm::lattice_view(Q);
+  for (int64_t n = 0; n < s->N; ++n) {
+    const size_t slot = static_cast<size_t>(s->slot_of[n]);
+    for (int q = 0; q < Q; ++q) {
+      const size_t i = static_cast<size_t>(q) * s->npad + slot;
+      double v;
+      if (s->prec == 0) {
+        std::memcpy(&v, raw.data() + 8 * i, 8);
+      } else if (s->prec == 1) {
+        float x;
+        std::memcpy(&x, raw.data() + 4 * i, 4);
+        v = double(x) + L.w[q];
+      } else {
+        uint16_t h;
+        std::memcpy(&h, raw.data() + 2 * i, 2);
+        v = double(half_to_float(h)) + L.w[q];
+      }
+      fs[n * Q + q] = v;
+    }
+  }
+  return LBM_OK;
+}
+
+int lbm_get_wss(lbm_solver* s, double* wss, double* acc_tau, double* acc_mag) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  const int64_t sp = s->n_special_pad;
+  std::vector<double> h(static_cast<size_t>(5 * sp));
+  if (sp) {
+    HIP_TRY(hipMemcpyAsync(h.data(), s->wss, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+  }
+  for (size_t i = 0; i < s->nw_slots.size(); ++i) {
+    const int64_t slot = s->nw_slots[i];
+    if (wss) wss[i] = h[slot];
+    if (acc_tau)
+      for (int a = 0; a < 3; ++a) acc_tau[3 * i + a] = h[(1 + a) * sp + slot];
+    if (acc_mag) acc_mag[i] = h[4 * sp + slot];
+  }
+  return LBM_OK;
+}
+
+int lbm_get_osi_tawss(lbm_solver* s, double* osi, double* tawss) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  const size_t n = s->nw_slots.size();
+  std::vector<double> acc(3 * n), mag(n);
+  int rc = lbm_get_wss(s, nullptr, acc.data(), mag.data());
+  if (rc) return rc;
+  const double period = 0.9;  // Pulsatile::period (pulsatile.h:53)
+  for (size_t i = 0; i < n; ++i) {
+    // Simulation::ComputeOSIAndTAWSS (Simulation.cpp:813-824).
+    const double t = mag[i] / period;
+    const double norm = sqrt(acc[3 * i] * acc[3 * i] + acc[3 * i + 1] * acc[3 * i + 1] + acc[3 * i + 2] * acc[3 * i + 2]);
+    if (osi) osi[i] = 0.5 * (1.0 - (norm / mag[i]));
+    if (tawss) tawss[i] = t * s->prm.dt;
+  }
+  return LBM_OK;
+}
+
+int lbm_get_macroscopic(lbm_solver* s, double* rho, double* u) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  const int Q = s->Q;
+  std::vector<double> f(static_cast<size_t>(s->N) * Q);
+  int rc = lbm_get_populations(s, f.data());
+  if (rc) return rc;
+  const lbm::LatticeView L = lbm::lattice_view(Q);
+  for (int64_t n = 0; n < s->N; ++n) {
+    const double* fn = f.data() + n * Q;
+    double r = 0.0;
+    for (int q = 0; q < Q; ++q) r += fn[q];
+    double jx = 0.0, jy = 0.0, jz = 0.0;
+    for (int q = 0; q < Q; ++q) {
+      jx += L.cx[q] * fn[q];
+      jy += L.cy[q] * fn[q];
+      jz += L.cz[q] * fn[q];
+    }
+    const bool wall0 = s->kind[n] == KIND_WALL && s->prm.wall_mode == LBM_WALL_REFERENCE;
+    if (rho) rho[n] = r;
+    if (u) {
+      u[3 * n] = wall0 ? 0.0 : jx / r;
+      u[3 * n + 1] = wall0 ? 0.0 : jy / r;
+      u[3 * n + 2] = wall0 ? 0.0 : jz / r;
+    }
+  }
+  return LBM_OK;
+}
+
+int lbm_reduce(lbm_solver* s, lbm_stats* out) {
+  if (!s || !out) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  StepArgs a = args_for(s, s->cur);
+  HIP_TRY(launch_moments(s->Q, s->prec, a, s->partials, s->red_out, s->stream));
+  HIP_TRY(launch_wss_stats(a, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
+  double h[6];
+  HIP_TRY(hipMemcpyAsync(h, s->red_out, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->mass = h[0];
+  out->momentum[0] = h[1];
+  out->momentum[1] = h[2];
+  out->momentum[2] = h[3];
+  out->wss_l2 = sqrt(h[4]);
+  out->wss_max = h[5];
+  out->iteration = s->iter;
+  out->time = s->total_runtime;
+  return LBM_OK;
+}
+
+int lbm_synchronize(lbm_solver* s) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_get_solver_info(const lbm_solver* s, lbm_solver_info* o) {
+  if (!s || !o) return fail(LBM_ERR_ARG, "null argument");
+  std::memset(o, 0, sizeof(*o));
+  o->n_nodes = s->N;
+  o->n_special = s->n_special;
+  o->n_bulk = s->n_bulk;
+  o->n_pad = s->npad;
+  o->device_bytes = static_cast<int64_t>(s->device_bytes);
+  o->q = s->Q;
+  o->precision = s->prec;
+  o->bytes_per_update = 2.0 * s->Q * double(s->esize);
+  return LBM_OK;
+}
+
+void lbm_destroy(lbm_solver* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->prm.device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  free_all(s);
+  delete s;
+}
+
+}  // extern "C"

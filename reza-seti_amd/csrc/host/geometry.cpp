@@ -1,0 +1,565 @@
+// Voxeliser, classifier and table builder (see geometry.h for the reference
+// map). All floating-point predicates are written with the reference's
+// operation order; this file must be compiled with -ffp-contract=off and
+// without fast-math so that the host evaluates them exactly like a plain
+// IEEE-754 binary64 build of the reference.
+#include "geometry.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+#include <thread>
+#include <unordered_set>
+
+#include "../lattice.h"
+
+namespace lbm {
+
+namespace {
+
+int resolve_threads(int n) {
+  if (n > 0) return n;
+  unsigned h = std::thread::hardware_concurrency();
+  return h == 0 ? 1 : static_cast<int>(std::min(h, 64u));
+}
+
+// Static-chunked parallel loop over [0, n).
+template <class F>
+void parallel_for(int64_t n, int nthreads, F&& body) {
+  nthreads = resolve_threads(nthreads);
+  if (nthreads <= 1 || n < 2) {
+    for (int64_t i = 0; i < n; ++i) body(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  const int64_t grain = std::max<int64_t>(1, n / (int64_t(nthreads) * 16));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nthreads; ++t)
+    pool.emplace_back([&]() {
+      for (;;) {
+        const int64_t a = next.fetch_add(grain);
+        if (a >= n) break;
+        const int64_t b = std::min(n, a + grain);
+        for (int64_t i = a; i < b; ++i) body(i);
+      }
+    });
+  for (auto& th : pool) th.join();
+}
+
+// Geometry::isPointInsideTriangle (ComplexGeometry.cpp:208-262) for the +z
+// ray direction used by labelPoints.
+bool inside_triangle_zray(double x, double y, double z, const Triangle& T) {
+  const double x1 = T.x1, y1 = T.y1, z1 = T.z1;
+  const double x2 = T.x2, y2 = T.y2, z2 = T.z2;
+  const double x3 = T.x3, y3 = T.y3, z3 = T.z3;
+  const double dirX = 0.0, dirY = 0.0, dirZ = 1.0;
+  const double uX = x2 - x1, uY = y2 - y1, uZ = z2 - z1;
+  const double vX = x3 - x1, vY = y3 - y1, vZ = z3 - z1;
+  const double nX = uY * vZ - uZ * vY;
+  const double nY = uZ * vX - uX * vZ;
+  const double nZ = uX * vY - uY * vX;
+  const double nLengthSq = nX * nX + nY * nY + nZ * nZ;
+  const double tol = 1e-16;
+  if (nLengthSq < tol * tol) return false;
+  const double w0X = x - x1, w0Y = y - y1, w0Z = z - z1;
+  const double a = -(nX * w0X + nY * w0Y + nZ * w0Z);
+  const double b = nX * dirX + nY * dirY + nZ * dirZ;
+  if (std::fabs(b) < tol) return false;
+  const double pos = a / b;
+  const double ix = x + pos * dirX;
+  const double iy = y + pos * dirY;
+  const double iz = z + pos * dirZ;
+  const double uu = uX * uX + uY * uY + uZ * uZ;
+  const double uv = uX * vX + uY * vY + uZ * vZ;
+  const double vv = vX * vX + vY * vY + vZ * vZ;
+  const double wX = ix - x1, wY = iy - y1, wZ = iz - z1;
+  const double wu = wX * uX + wY * uY + wZ * uZ;
+  const double wv = wX * vX + wY * vY + wZ * vZ;
+  const double D = uv * uv - uu * vv;
+  const double s = (uv * wv - vv * wu) / D;
+  if (s < 0.0 || s > 1.0) return false;
+  const double t = (uv * wu - uu * wv) / D;
+  if (t < 0.0 || (s + t) > 1.0) return false;
+  return true;
+}
+
+// One triangle of Geometry::countIntersectinsTriangle (ComplexGeometry.cpp:265-310)
+// for a ray from (px,py,pz) to (ex,ey,ez).
+inline bool segment_counts(double px, double py, double pz, double ex, double ey, double ez,
+                           const Triangle& T) {
+  const double threshold = 1e-16;
+  const double dx = ex - px, dy = ey - py, dz = ez - pz;
+  const double numerator = (T.x1 - px) * T.nx + (T.y1 - py) * T.ny + (T.z1 - pz) * T.nz;
+  const double denominator = dx * T.nx + dy * T.ny + dz * T.nz;
+  if (std::fabs(denominator) < threshold) return false;
+  const double t = numerator / denominator;
+  if (!(t >= 0.0 && t <= 1.0)) return false;
+  const double ix = px + t * dx;
+  const double iy = py + t * dy;
+  const double iz = pz + t * dz;
+  return inside_triangle_zray(ix, iy, iz, T);
+}
+
+// Ray end points of labelPoints case 0 plus adjustLineCoordinates case 0
+// (ComplexGeometry.cpp:314-336, 463-468).
+struct Ray {
+  double sx, sy, sz, ex, ey, ez;
+};
+
+inline double adjust_xy(double v, double lo, double hi) {
+  const double off = 0.01;
+  if (v == lo) v += off;
+  if (v == hi) v -= off;
+  return v;
+}
+
+inline Ray make_ray(const GridSpec& g, double x, double y, double z) {
+  const BBox& b = g.bbox;
+  const double rayLength = b.max_z - b.min_z;
+  Ray r;
+  r.sx = adjust_xy(x, b.min_x, b.max_x);
+  r.sy = adjust_xy(y, b.min_y, b.max_y);
+  r.ex = r.sx;
+  r.ey = r.sy;
+  r.sz = z;
+  r.ez = z + rayLength + 0.1;
+  if (r.sz == b.min_z) r.sz += 0.01;
+  if (r.ez == b.max_z) r.ez -= 0.01;
+  return r;
+}
+
+// A triangle that can never count: zero file-normal z (denominator is exactly
+// zero for a +z ray) or a degenerate vertex set (rejected first thing in the
+// inside test).
+bool never_counts(const Triangle& T) {
+  if (T.nz == 0.0) return true;
+  const double uX = T.x2 - T.x1, uY = T.y2 - T.y1, uZ = T.z2 - T.z1;
+  const double vX = T.x3 - T.x1, vY = T.y3 - T.y1, vZ = T.z3 - T.z1;
+  const double nX = uY * vZ - uZ * vY, nY = uZ * vX - uX * vZ, nZ = uX * vY - uY * vX;
+  const double l2 = nX * nX + nY * nY + nZ * nZ;
+  if (l2 < 1e-16 * 1e-16) return true;
+  if (std::fabs(nZ) < 1e-16) return true;
+  return false;
+}
+
+}  // namespace
+
+bool ray_counts_triangle(const GridSpec& g, double x, double y, double z, const Triangle& t) {
+  const Ray r = make_ray(g, x, y, z);
+  return segment_counts(r.sx, r.sy, r.sz, r.ex, r.ey, r.ez, t);
+}
+
+BBox bounding_box(const std::vector<Triangle>& tris) {
+  // Geometry::GetBoundingbox (ComplexGeometry.cpp:90-104), same min/max nesting.
+  const double inf = std::numeric_limits<double>::infinity();
+  double mnx = inf, mny = inf, mnz = inf, mxx = -inf, mxy = -inf, mxz = -inf;
+  for (const Triangle& t : tris) {
+    mnx = std::min(mnx, std::min(t.x1, std::min(t.x2, t.x3)));
+    mny = std::min(mny, std::min(t.y1, std::min(t.y2, t.y3)));
+    mnz = std::min(mnz, std::min(t.z1, std::min(t.z2, t.z3)));
+    mxx = std::max(mxx, std::max(t.x1, std::max(t.x2, t.x3)));
+    mxy = std::max(mxy, std::max(t.y1, std::max(t.y2, t.y3)));
+    mxz = std::max(mxz, std::max(t.z1, std::max(t.z2, t.z3)));
+  }
+  return {mnx, mxx, mny, mxy, mnz, mxz};
+}
+
+GridSpec make_grid(const BBox& bb, double grid_size) {
+  // DistributeTriangles doubles the spacing (ComplexGeometry.cpp:157) before
+  // generatePointsInBoundingBox derives nx = ceil(range / spacing) (:124-128).
+  GridSpec g;
+  g.bbox = bb;
+  double spacing = grid_size;
+  spacing *= 2;
+  g.spacing = spacing;
+  const int nx = static_cast<int>(std::ceil((bb.max_x - bb.min_x) / spacing));
+  const int ny = static_cast<int>(std::ceil((bb.max_y - bb.min_y) / spacing));
+  const int nz = static_cast<int>(std::ceil((bb.max_z - bb.min_z) / spacing));
+  g.NX = nx + 1;
+  g.NY = ny + 1;
+  g.NZ = nz + 1;
+  return g;
+}
+
+bool label_points(const GridSpec& g, const std::vector<Triangle>& tris, std::vector<uint8_t>& fluid,
+                  int nthreads, int verify_stride, std::string* err) {
+  const int NX = g.NX, NY = g.NY, NZ = g.NZ;
+  if (NX <= 0 || NY <= 0 || NZ <= 0) {
+    if (err) *err = "empty grid";
+    return false;
+  }
+  fluid.assign(static_cast<size_t>(g.npoints()), 0);
+
+  // Ray x (resp. y) of every column, after the bounding-box offsets.
+  std::vector<double> rx(NX), ry(NY);
+  for (int i = 0; i < NX; ++i) rx[i] = adjust_xy(g.px(i), g.bbox.min_x, g.bbox.max_x);
+  for (int j = 0; j < NY; ++j) ry[j] = adjust_xy(g.py(j), g.bbox.min_y, g.bbox.max_y);
+  std::vector<std::pair<double, int>> sx(NX), sy(NY);
+  for (int i = 0; i < NX; ++i) sx[i] = {rx[i], i};
+  for (int j = 0; j < NY; ++j) sy[j] = {ry[j], j};
+  std::sort(sx.begin(), sx.end());
+  std::sort(sy.begin(), sy.end());
+
+  // Candidate filter: a triangle can only count for a ray whose (x, y) lies in
+  // the triangle's xy bounding box. The box is widened by a margin far above
+  // the rounding error of the barycentric test, so no counting pair is lost.
+  auto cols_of = [&](const Triangle& T, std::vector<int>& is, std::vector<int>& js) {
+    is.clear();
+    js.clear();
+    const double bx0 = std::min(T.x1, std::min(T.x2, T.x3)), bx1 = std::max(T.x1, std::max(T.x2, T.x3));
+    const double by0 = std::min(T.y1, std::min(T.y2, T.y3)), by1 = std::max(T.y1, std::max(T.y2, T.y3));
+    const double ext = std::max(bx1 - bx0, by1 - by0);
+    const double scale = std::max({std::fabs(bx0), std::fabs(bx1), std::fabs(by0), std::fabs(by1), 1.0});
+    const double m = 1e-6 * ext + 1e-9 * scale;
+    auto lo_x = std::lower_bound(sx.begin(), sx.end(), std::make_pair(bx0 - m, -1));
+    for (auto it = lo_x; it != sx.end() && it->first <= bx1 + m; ++it) is.push_back(it->second);
+    auto lo_y = std::lower_bound(sy.begin(), sy.end(), std::make_pair(by0 - m, -1));
+    for (auto it = lo_y; it != sy.end() && it->first <= by1 + m; ++it) js.push_back(it->second);
+  };
+
+  const int64_t ncol = int64_t(NX) * NY;
+  std::vector<int64_t> start(ncol + 1, 0);
+  {
+    std::vector<int> is, js;
+    for (const Triangle& T : tris) {
+      if (never_counts(T)) continue;
+      cols_of(T, is, js);
+      for (int i : is)
+        for (int j : js) start[int64_t(i) * NY + j + 1]++;
+    }
+  }
+  for (int64_t c = 0; c < ncol; ++c) start[c + 1] += start[c];
+  std::vector<int32_t> cand(static_cast<size_t>(start[ncol]));
+  {
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    std::vector<int> is, js;
+    for (size_t ti = 0; ti < tris.size(); ++ti) {
+      const Triangle& T = tris[ti];
+      if (never_counts(T)) continue;
+      cols_of(T, is, js);
+      for (int i : is)
+        for (int j : js) cand[fill[int64_t(i) * NY + j]++] = static_cast<int32_t>(ti);
+    }
+  }
+
+  const double rayLength = g.bbox.max_z - g.bbox.min_z;
+  parallel_for(ncol, nthreads, [&](int64_t c) {
+    const int i = static_cast<int>(c / NY), j = static_cast<int>(c % NY);
+    const double x = rx[i], y = ry[j];
+    const int64_t a = start[c], b = start[c + 1];
+    uint8_t* col = fluid.data() + c * NZ;
+    for (int k = 0; k < NZ; ++k) {
+      double sz = g.pz(k);
+      double ez = g.pz(k) + rayLength + 0.1;
+      if (sz == g.bbox.min_z) sz += 0.01;
+      if (ez == g.bbox.max_z) ez -= 0.01;
+      int count = 0;
+      for (int64_t e = a; e < b; ++e)
+        count += segment_counts(x, y, sz, x, y, ez, tris[cand[e]]) ? 1 : 0;
+      col[k] = (count % 2 == 1) ? 1 : 0;
+    }
+  });
+
+  if (verify_stride > 0) {
+    std::atomic<int64_t> bad{0};
+    const int64_t nver = (ncol + verify_stride - 1) / verify_stride;
+    parallel_for(nver, nthreads, [&](int64_t v) {
+      const int64_t c = v * verify_stride;
+      const int i = static_cast<int>(c / NY), j = static_cast<int>(c % NY);
+      for (int k = 0; k < NZ; ++k) {
+        int count = 0;
+        for (const Triangle& T : tris)
+          count += ray_counts_triangle(g, g.px(i), g.py(j), g.pz(k), T) ? 1 : 0;
+        if (((count % 2 == 1) ? 1 : 0) != fluid[c * NZ + k]) bad++;
+      }
+    });
+    if (bad.load() != 0) {
+      if (err) *err = "voxeliser candidate filter disagreed with the full scan on " +
+                      std::to_string(bad.load()) + " points";
+      return false;
+    }
+  }
+  return true;
+}
+
+std::array<double, 3> boundary_normal(double x, double y, double z, const std::vector<Triangle>& tris) {
+  // SubDomain::calculateBoundaryNormals body (Subdomain.cpp:519-570). The
+  // reference's std::pow(d, 2) is the exact square (compilers fold it to d*d).
+  double n0 = 0.0, n1 = 0.0, n2 = 0.0;
+  for (const Triangle& T : tris) {
+    const double ux = T.x2 - T.x1, uy = T.y2 - T.y1, uz = T.z2 - T.z1;
+    const double vx = T.x3 - T.x1, vy = T.y3 - T.y1, vz = T.z3 - T.z1;
+    double nx = uy * vz - uz * vy;
+    double ny = uz * vx - ux * vz;
+    double nz = ux * vy - uy * vx;
+    const double norm = std::sqrt(nx * nx + ny * ny + nz * nz);
+    if (norm > 0) {
+      nx /= norm;
+      ny /= norm;
+      nz /= norm;
+    }
+    const double area = 0.5 * norm;
+    const double cX = (T.x1 + T.x2 + T.x3) / 3.0;
+    const double cY = (T.y1 + T.y2 + T.y3) / 3.0;
+    const double cZ = (T.z1 + T.z2 + T.z3) / 3.0;
+    const double ddx = x - cX, ddy = y - cY, ddz = z - cZ;
+    const double distance = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    const double weight = area / (distance * distance);
+    n0 += nx * weight;
+    n1 += ny * weight;
+    n2 += nz * weight;
+  }
+  const double norm = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  if (norm > 0) {
+    n0 /= norm;
+    n1 /= norm;
+    n2 /= norm;
+  }
+  return {n0, n1, n2};
+}
+
+namespace {
+
+// Plane/range test of Geometry::getInOutBP (ComplexGeometry.cpp:583-597).
+bool in_plane_block(const PlaneSpec& s, double x, double y, double z) {
+  const long double tolerance = 2.5e-2;
+  if (s.plane == "x" && std::abs(x - s.coordinate) <= tolerance)
+    return y >= s.range_y[0] && y <= s.range_y[1] && z >= s.range_z[0] && z <= s.range_z[1];
+  if (s.plane == "y" && std::abs(y - s.coordinate) <= tolerance)
+    return x >= s.range_x[0] && x <= s.range_x[1] && z >= s.range_z[0] && z <= s.range_z[1];
+  if (s.plane == "z" && std::abs(z - s.coordinate) <= tolerance)
+    return x >= s.range_x[0] && x <= s.range_x[1] && y >= s.range_y[0] && y <= s.range_y[1];
+  return false;
+}
+
+bool check_block(const PlaneSpec& s, const char* what, std::string* err) {
+  if (s.range_x.size() < 2 || s.range_y.size() < 2 || s.range_z.size() < 2) {
+    if (err) *err = std::string(what) + " block has a range with fewer than 2 values";
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+bool build_from_labels(const GridSpec& g, const std::vector<uint8_t>& fluid,
+                       const std::vector<PlaneSpec>& inlets, const std::vector<PlaneSpec>& outlets,
+                       const std::vector<Triangle>& tris, const BuildOptions& o, Tables& t,
+                       std::string* err) {
+  if (o.Q != 19 && o.Q != 27) {
+    if (err) *err = "Q must be 19 or 27";
+    return false;
+  }
+  if (!(g.spacing > 2e-6)) {
+    // Index-set membership stands in for the reference's 1e-6 coordinate match
+    // (Utility.cpp:59-74); both agree only while grid points are > 2e-6 apart.
+    if (err) *err = "lattice pitch must exceed 2e-6";
+    return false;
+  }
+  for (const auto& s : inlets)
+    if (!check_block(s, "inlet", err)) return false;
+  for (const auto& s : outlets)
+    if (!check_block(s, "outlet", err)) return false;
+
+  const LatticeView L = lattice_view(o.Q);
+  const int NX = g.NX, NY = g.NY, NZ = g.NZ;
+  t = Tables();
+  t.grid = g;
+  t.Q = o.Q;
+  for (int a = 0; a < 3; ++a) t.periodic[a] = o.periodic[a];
+  t.fluid = fluid;
+  const int dims[3] = {NX, NY, NZ};
+
+  auto wrap = [&](int v, int axis, bool& ok) -> int {
+    if (v >= 0 && v < dims[axis]) return v;
+    if (o.periodic[axis]) return (v + dims[axis]) % dims[axis];
+    ok = false;
+    return v;
+  };
+
+  // Boundary points (Geometry::findBoundaryPoints, ComplexGeometry.cpp:517-564).
+  const int64_t P = g.npoints();
+  std::vector<uint8_t> is_bp(static_cast<size_t>(P), 0);
+  parallel_for(NX, o.nthreads, [&](int64_t ii) {
+    const int i = static_cast<int>(ii);
+    for (int j = 0; j < NY; ++j)
+      for (int k = 0; k < NZ; ++k) {
+        const int64_t gi = g.gidx(i, j, k);
+        if (!fluid[gi]) continue;
+        bool bp = false;
+        for (int dx = -1; dx <= 1 && !bp; ++dx)
+          for (int dy = -1; dy <= 1 && !bp; ++dy)
+            for (int dz = -1; dz <= 1 && !bp; ++dz) {
+              bool ok = true;
+              const int a = wrap(i + dx, 0, ok), b = wrap(j + dy, 1, ok), c = wrap(k + dz, 2, ok);
+              if (ok && !fluid[g.gidx(a, b, c)]) bp = true;
+            }
+        const bool at_face = (!o.periodic[0] && (i == 0 || i == NX - 1)) ||
+                             (!o.periodic[1] && (j == 0 || j == NY - 1)) ||
+                             (!o.periodic[2] && (k == 0 || k == NZ - 1));
+        if (at_face) bp = true;
+        is_bp[gi] = bp ? 1 : 0;
+      }
+  });
+
+  // Node kinds with CreateNodes precedence: inlet > first outlet > wall > fluid.
+  std::vector<int32_t> node_of(static_cast<size_t>(P), -1);
+  int64_t N = 0;
+  for (int64_t gi = 0; gi < P; ++gi)
+    if (fluid[gi]) node_of[gi] = static_cast<int32_t>(N++);
+  if (N >= (int64_t(1) << 31) - 1) {
+    if (err) *err = "more than 2^31-1 nodes";
+    return false;
+  }
+  t.N = N;
+  t.ijk.resize(3 * N);
+  t.kind.assign(N, KIND_FLUID);
+  t.outlet_id.assign(N, -1);
+  t.outlet_nodes.assign(outlets.size(), {});
+  t.outlet_normals.clear();
+  for (const auto& s : outlets) {
+    std::array<double, 3> n{0, 0, 0};
+    for (size_t a = 0; a < 3 && a < s.normal.size(); ++a) n[a] = s.normal[a];
+    t.outlet_normals.push_back(n);
+  }
+  if (!inlets.empty()) {
+    // Only the last inlet block's normal survives (ComplexGeometry.cpp:605).
+    const auto& nrm = inlets.back().normal;
+    if (nrm.size() < 3) {
+      if (err) *err = "inlet normal must have 3 components";
+      return false;
+    }
+    t.inlet_normal = {nrm[0], nrm[1], nrm[2]};
+    t.has_inlet_normal = true;
+  }
+  int64_t nbp = 0;
+  for (int i = 0; i < NX; ++i)
+    for (int j = 0; j < NY; ++j)
+      for (int k = 0; k < NZ; ++k) {
+        const int64_t gi = g.gidx(i, j, k);
+        const int32_t n = node_of[gi];
+        if (n < 0) continue;
+        t.ijk[3 * n] = i;
+        t.ijk[3 * n + 1] = j;
+        t.ijk[3 * n + 2] = k;
+        if (!is_bp[gi]) continue;
+        ++nbp;
+        const double x = g.px(i), y = g.py(j), z = g.pz(k);
+        bool inl = false;
+        for (const auto& s : inlets) inl = inl || in_plane_block(s, x, y, z);
+        if (inl) {
+          t.kind[n] = KIND_INLET;
+          t.inlet_nodes.push_back(n);
+          continue;
+        }
+        int oid = -1;
+        for (size_t s = 0; s < outlets.size(); ++s)
+          if (in_plane_block(outlets[s], x, y, z)) {
+            oid = static_cast<int>(s);
+            break;
+          }
+        if (oid >= 0) {
+          t.kind[n] = KIND_OUTLET;
+          t.outlet_id[n] = oid;
+          t.outlet_nodes[oid].push_back(n);
+          continue;
+        }
+        t.kind[n] = KIND_WALL;
+        t.wall_nodes.push_back(n);
+      }
+  t.n_boundary_points = nbp;
+
+  // Push connectivity (SubDomain::Connectivity, Subdomain.cpp:447-471).
+  const int Q = L.Q;
+  t.nb.assign(static_cast<size_t>(Q) * N, -1);
+  parallel_for(N, o.nthreads, [&](int64_t n) {
+    const int i = t.ijk[3 * n], j = t.ijk[3 * n + 1], k = t.ijk[3 * n + 2];
+    for (int q = 0; q < Q; ++q) {
+      bool ok = true;
+      const int a = wrap(i + L.cx[q], 0, ok), b = wrap(j + L.cy[q], 1, ok), c = wrap(k + L.cz[q], 2, ok);
+      t.nb[static_cast<size_t>(q) * N + n] = ok ? node_of[g.gidx(a, b, c)] : -1;
+    }
+  });
+
+  // Node behind an inlet/outlet along its block normal: the outlet copy source
+  // of Simulation::ApplyOutletBc (Simulation.cpp:650-741, else-if order
+  // +x,-x,+y,-y,+z,-z) and the interior neighbour used by extension BCs.
+  t.bc_nbr.assign(N, -1);
+  auto behind = [&](int32_t n, const std::array<double, 3>& nv) -> int32_t {
+    int di = 0, dj = 0, dk = 0;
+    if (nv[0] > 0) di = -1;
+    else if (nv[0] < 0) di = 1;
+    else if (nv[1] > 0) dj = -1;
+    else if (nv[1] < 0) dj = 1;
+    else if (nv[2] > 0) dk = -1;
+    else if (nv[2] < 0) dk = 1;
+    else return -1;
+    bool ok = true;
+    const int a = wrap(t.ijk[3 * n] + di, 0, ok), b = wrap(t.ijk[3 * n + 1] + dj, 1, ok),
+              c = wrap(t.ijk[3 * n + 2] + dk, 2, ok);
+    if (!ok) {
+      t.upstream_out_of_box++;
+      return -1;
+    }
+    return node_of[g.gidx(a, b, c)];
+  };
+  for (size_t s = 0; s < t.outlet_nodes.size(); ++s)
+    for (int32_t n : t.outlet_nodes[s]) t.bc_nbr[n] = behind(n, t.outlet_normals[s]);
+  if (t.has_inlet_normal)
+    for (int32_t n : t.inlet_nodes) t.bc_nbr[n] = behind(n, t.inlet_normal);
+
+  // Near-wall set (SubDomain::IdentifyNextToWallLattices, Subdomain.cpp:475-493):
+  // plain-fluid neighbours of wall nodes. The reference keeps them in an
+  // unordered_set (output-only order); here they are sorted by node index.
+  {
+    std::vector<uint8_t> mark(N, 0);
+    for (int32_t w : t.wall_nodes)
+      for (int q = 0; q < Q; ++q) {
+        const int32_t nbr = t.nb[static_cast<size_t>(q) * N + w];
+        if (nbr >= 0 && t.kind[nbr] == KIND_FLUID) mark[nbr] = 1;
+      }
+    t.nearwall.clear();
+    for (int64_t n = 0; n < N; ++n)
+      if (mark[n]) t.nearwall.push_back(static_cast<int32_t>(n));
+  }
+  t.normals.assign(3 * t.nearwall.size(), 0.0);
+  if (o.compute_normals) {
+    parallel_for(static_cast<int64_t>(t.nearwall.size()), o.nthreads, [&](int64_t a) {
+      const int32_t n = t.nearwall[a];
+      const auto nv = boundary_normal(g.px(t.ijk[3 * n]), g.py(t.ijk[3 * n + 1]), g.pz(t.ijk[3 * n + 2]), tris);
+      t.normals[3 * a] = nv[0];
+      t.normals[3 * a + 1] = nv[1];
+      t.normals[3 * a + 2] = nv[2];
+    });
+  }
+  return true;
+}
+
+bool build_from_config(const SimulationParameters& p, const std::string& base_dir, const BuildOptions& o,
+                       Tables& t, std::vector<Triangle>* tris_out, std::string* err) {
+  std::string path = p.stlFile;
+  if (!path.empty() && path[0] != '/' && !base_dir.empty()) path = base_dir + "/" + path;
+  std::vector<Triangle> tris;
+  if (!readSTLASCII(path, tris)) {
+    if (err) *err = "cannot open STL file " + path;
+    return false;
+  }
+  if (tris.empty()) {
+    if (err) *err = "STL file has no facets: " + path;
+    return false;
+  }
+  if (!(p.dx > 0)) {
+    if (err) *err = "grid_size must be positive";
+    return false;
+  }
+  const GridSpec g = make_grid(bounding_box(tris), p.dx);
+  std::vector<uint8_t> fluid;
+  if (!label_points(g, tris, fluid, o.nthreads, o.verify_stride, err)) return false;
+  if (!build_from_labels(g, fluid, p.inlets, p.outlets, tris, o, t, err)) return false;
+  if (tris_out) tris_out->swap(tris);
+  return true;
+}
+
+}  // namespace lbm

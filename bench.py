@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused collide-stream path on one MI355X.
+
+Workload: the D3Q19 precision-study vessel (Y-bifurcation, parent radius 24,
+daughters radius 18 at 35 degrees, 3-mode wall perturbation, ~3M nodes),
+Womersley inlet (Wo = 4, period 4000 steps), zero-pressure outlets, half-way
+bounce-back walls. The headline `value` is the whole-job MLUPS of the
+FP32-shifted path; FP64 and FP16-storage runs of the same workload are
+reported beside it with their velocity / WSS error against FP64.
+
+A "step" is one fused kernel launch = one reference iteration over all nodes.
+Inputs are resident in HBM when the timed region starts.
+
+    python bench.py [--gpus 1] [--steps 8000] [--warmup 100]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "reza-seti_amd"))
+
+import lbm  # noqa: E402
+import workloads  # noqa: E402
+
+METRIC = "MLUPS & %HBM roofline, FP32 vs FP64 at ≤1e-4 WSS err; adjoint/forward time"
+PEAK_HBM_GBS = 8000.0
+WO, PERIOD, U_MEAN, U_AMP = 4.0, 4000, 0.05, 0.025
+
+
+def womersley_basis(r):
+    """Three inlet modes: mean Poiseuille (mean 1) and the real / imaginary
+    parts of the Womersley profile normalised to unit mean over the inlet
+    nodes, so u = U_mean b0 + U_amp (b1 cos wt + b2 sin wt)."""
+    from scipy.special import jv
+
+    r = np.clip(r, 0.0, 1.0)
+    b0 = 2.0 * (1.0 - r * r)
+    b0 /= b0.mean()
+    a = (1j ** 1.5) * WO
+    P = 1.0 - jv(0, a * r) / jv(0, a)
+    P /= P.mean()
+    return np.stack([b0, P.real, -P.imag])
+
+
+def coefficients(t0, n):
+    t = np.arange(t0, t0 + n, dtype=np.float64)
+    w = 2.0 * math.pi / PERIOD
+    return np.stack([np.full(n, U_MEAN), U_AMP * np.cos(w * t), U_AMP * np.sin(w * t)], 1)
+
+
+def build_geometry(workdir):
+    t0 = time.time()
+    cfg, info = workloads.c1_bifurcation(workdir)
+    t1 = time.time()
+    geom = lbm.Geometry.from_config(cfg, normals=2)
+    t2 = time.time()
+    ijk, kind, _ = geom.nodes()
+    b = geom.info.bbox
+    inl = np.where(kind == lbm.KIND_INLET)[0]
+    x = b[0] + ijk[inl, 0] * geom.info.spacing
+    y = b[2] + ijk[inl, 1] * geom.info.spacing
+    r = np.hypot(x - info["parent_axis"][0], y - info["parent_axis"][1]) / info["parent_radius"]
+    print(f"# geometry: {info['n_triangles']} triangles (mesh {t1 - t0:.1f}s), {geom.n} nodes, "
+          f"grid {tuple(geom.info.grid)}, tables {t2 - t1:.1f}s", file=sys.stderr, flush=True)
+    return geom, womersley_basis(r), info
+
+
+def tau_for():
+    R = 24.0
+    nu = R * R * (2 * math.pi / PERIOD) / (WO * WO)
+    return 3.0 * nu + 0.5
+
+
+def run_precision(geom, basis, prec, steps, warmup):
+    s = lbm.Solver(geom, precision=prec, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NEE_VELOCITY,
+                   outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=tau_for(), dt=1e-5, parity_checks=0, wss_every=1)
+    s.set_inlet_profile(basis)
+    if warmup:
+        s.step_coefficients(coefficients(0, warmup))
+    s.synchronize()
+    coef = coefficients(warmup, steps)
+    t0 = time.perf_counter()
+    s.step_coefficients(coef)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms, n = s.last_batch_ms()
+    assert n == steps
+    return s, wall, ev_ms
+
+
+def cpu_baseline(geom, budget_s=12.0):
+    """The oracle (FP64 restatement of the reference step, OpenMP) on the host
+    cores, on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    ncores = min(16, os.cpu_count() or 1)
+    ijk, kind, _ = geom.nodes()
+    ref = oracle.Reference.from_geometry(geom, tau=tau_for(), dt=1e-5, wall_mode=1, inlet_mode=1, outlet_mode=1)
+    t0 = time.perf_counter()
+    ref.step(np.full(2, U_MEAN), nthreads=ncores)
+    per = (time.perf_counter() - t0) / 2
+    n = int(max(3, min(200, budget_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    ref.step(np.full(n, U_MEAN), nthreads=ncores)
+    dt = time.perf_counter() - t0
+    return {"value": round(geom.n * n / dt / 1e6, 3), "unit": "MLUPS", "cores": ncores, "kind": "port",
+            "sample": f"{n} steps of the same {geom.n}-node bifurcation, oracle FP64 restatement of the "
+                      f"reference step (bounce-back walls, NEE inlet/outlet), OpenMP on {ncores} host threads"}
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / d) if d > 0 else float("nan")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precisions", default="f32,f64,f16")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the whole lattice on a single MI355X "
+              "(no sharding or collectives); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    with tempfile.TemporaryDirectory() as wd:
+        geom, basis, info = build_geometry(wd)
+    N = geom.n
+    precs = {"f32": lbm.PREC_F32, "f64": lbm.PREC_F64, "f16": lbm.PREC_F16}
+    runs = {}
+    for name in args.precisions.split(","):
+        s, wall, ev_ms = run_precision(geom, basis, precs[name], args.steps, args.warmup)
+        runs[name] = dict(solver=s, wall=wall, ev_ms=ev_ms, mlups=N * args.steps / wall / 1e6)
+        print(f"# {name}: {runs[name]['mlups']:.0f} MLUPS (wall {wall:.3f}s, events {ev_ms:.1f}ms)",
+              file=sys.stderr, flush=True)
+
+    head = runs["f32"]
+    sinfo = head["solver"].info()
+    bytes_per_launch = N * sinfo.bytes_per_update
+    per_launch_s = head["ev_ms"] / 1e3 / args.steps
+    achieved = bytes_per_launch / per_launch_s / 1e9
+    out = {
+        "metric": METRIC, "value": round(head["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(head["wall"] * 1e3 / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "C1 precision-study Y-bifurcation, D3Q19, FP32-shifted populations",
+                   "nodes": N, "special_nodes": int(sinfo.n_special), "triangles": info["n_triangles"],
+                   "inlet": f"Womersley Wo={WO} period {PERIOD} steps, U_mean {U_MEAN}",
+                   "outlet": "NEE pressure rho=1", "walls": "half-way bounce-back", "tau": round(tau_for(), 6)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "bytes_per_launch_algorithmic": bytes_per_launch,
+                     "launch_us": round(per_launch_s * 1e6, 3)},
+    }
+    if "f64" in runs:
+        out["fp64_mlups"] = round(runs["f64"]["mlups"], 1)
+        out["fp32_over_fp64"] = round(head["mlups"] / runs["f64"]["mlups"], 3)
+        ref = runs["f64"]["solver"]
+        w64, _, m64 = ref.wss()
+        _, u64 = ref.macroscopic()
+        for name in ("f32", "f16"):
+            if name not in runs:
+                continue
+            w, _, m = runs[name]["solver"].wss()
+            _, u = runs[name]["solver"].macroscopic()
+            out[f"{name}_wss_rel_l2"] = rel_l2(w, w64)
+            out[f"{name}_tawss_rel_l2"] = rel_l2(m, m64)
+            out[f"{name}_velocity_rel_l2"] = rel_l2(u, u64)
+    if "f16" in runs:
+        out["fp16_mlups"] = round(runs["f16"]["mlups"], 1)
+    out["adjoint_over_forward"] = None
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(geom)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

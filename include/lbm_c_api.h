@@ -109,7 +109,8 @@ typedef struct {
   int32_t periodic[3];     /* extension: periodic axes (reference: none) */
   int32_t nthreads;        /* host threads, 0 = all */
   int32_t verify_stride;   /* >0: re-check every n-th column against all triangles */
-  int32_t compute_normals; /* near-wall normals (Subdomain.cpp:499-572) */
+  int32_t compute_normals; /* near-wall normals (Subdomain.cpp:499-572): 0 none, 1 host, 2 GPU */
+  int32_t device;          /* HIP device for compute_normals = 2 */
 } lbm_geometry_options;
 
 void lbm_geometry_options_default(lbm_geometry_options* o);
@@ -253,6 +254,11 @@ int lbm_reduce(lbm_solver* s, lbm_stats* out);
 /* Run nsteps and report the device time of the step kernels, measured with
  * HIP events on the solver's stream (milliseconds, all steps). */
 int lbm_step_timed(lbm_solver* s, int32_t nsteps, double* ms);
+
+/* Device time (HIP events on the solver's stream) of the most recent
+ * lbm_step / lbm_step_coefficients / lbm_step_timed batch, and its step
+ * count. Waits for that batch to finish. */
+int lbm_last_batch_time(lbm_solver* s, double* ms, int32_t* nsteps);
 
 int lbm_synchronize(lbm_solver* s);
 void lbm_destroy(lbm_solver* s);

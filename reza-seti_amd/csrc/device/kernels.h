@@ -84,4 +84,8 @@ hipError_t launch_moments(int Q, int precision, const StepArgs& a, double* parti
 // Sum of |tau|^2 and max |tau| over near-wall slots, 2 FP64 values to out2.
 hipError_t launch_wss_stats(const StepArgs& a, double* partials, double* out2, hipStream_t st);
 
+// Deterministic fold of nblk partial rows of nv doubles (nv = 4: sums;
+// nv = 2 with maxlast: sum, max) into out.
+hipError_t launch_finish(const double* partials, int32_t nblk, int nv, bool maxlast, double* out, hipStream_t st);
+
 }  // namespace lbm

@@ -15,6 +15,7 @@
 
 #include "../capi_internal.h"
 #include "../lattice.h"
+#include "columns.h"
 #include "kernels.h"
 
 using namespace lbm;
@@ -100,8 +101,26 @@ struct lbm_solver {
   bool stepped = false;              // a step ran since the last set/create
   std::vector<double> last_set;      // f given to the last set (node-major), empty = w
   int n_modes = 1;
+  int32_t batch_steps = 0;           // steps of the last lbm_step* call (timed by ev0/ev1)
   double last_coef[MAX_MODES] = {0};
   StepArgs base{};
+
+  // Column-offset family (half-way bounce-back walls), see columns.h.
+  int family = 0;                    // 0: reference rules (kernels.hip), 1: columns.hip
+  uint16_t* idx16 = nullptr;
+  int32_t* cbase = nullptr;
+  int32_t* irr_src = nullptr;
+  uint32_t* irr_kind = nullptr;
+  int32_t* irr_bcn = nullptr;
+  int32_t* irr_inlet = nullptr;
+  int32_t* bslots = nullptr;
+  int32_t* nw_slot = nullptr;
+  int32_t* nw_src = nullptr;
+  double* nw_normal = nullptr;
+  double* nw_wss = nullptr;
+  int64_t n_irr = 0;
+  ColArgs cbase_args{};
+  ColWssArgs wss_args{};
 };
 
 namespace {
@@ -126,7 +145,8 @@ int dalloc_raw(lbm_solver* s, void** p, size_t bytes) {
 void free_all(lbm_solver* s) {
   void* ptrs[] = {s->fs[0], s->fs[1], s->fprev[0], s->fprev[1], s->src, s->sflags, s->bc_nbr, s->inlet_slot,
                   s->inlet_basis, s->normal, s->rho_u, s->wss, s->porosity, s->err, s->f_tmp, s->partials,
-                  s->red_out};
+                  s->red_out, s->idx16, s->cbase, s->irr_src, s->irr_kind, s->irr_bcn, s->irr_inlet,
+                  s->bslots, s->nw_slot, s->nw_src, s->nw_normal, s->nw_wss};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -143,6 +163,15 @@ StepArgs args_for(lbm_solver* s, int in) {
   a.fprev_in = s->fprev[in];
   a.fprev_out = s->fprev[1 - in];
   return a;
+}
+
+ColArgs col_args_for(lbm_solver* s, int in) {
+  ColArgs c = s->cbase_args;
+  c.fs_in = s->fs[in];
+  c.fs_out = s->fs[1 - in];
+  c.porosity = s->porosity;
+  c.n_modes = s->n_modes;
+  return c;
 }
 
 int ensure_tmp(lbm_solver* s) {
@@ -169,9 +198,14 @@ int init_state(lbm_solver* s, const double* f_host) {
     HIP_TRY(hipStreamSynchronize(s->stream));
   }
   a.wss_on = 0;
-  HIP_TRY(launch_lbm(s->Q, s->prec, KMODE_INIT, s->porosity || s->prm.force[0] != 0.0 ||
-                                                    s->prm.force[1] != 0.0 || s->prm.force[2] != 0.0,
-                     a, s->stream));
+  const bool frc = s->porosity || s->prm.force[0] != 0.0 || s->prm.force[1] != 0.0 || s->prm.force[2] != 0.0;
+  if (s->family == 1) {
+    ColArgs c = col_args_for(s, 1 - s->cur);
+    c.f_direct = a.f_direct;
+    HIP_TRY(launch_columns(s->Q, s->prec, KMODE_INIT, frc, c, s->stream));
+  } else {
+    HIP_TRY(launch_lbm(s->Q, s->prec, KMODE_INIT, frc, a, s->stream));
+  }
   s->stepped = false;
   if (f_host) s->last_set.assign(f_host, f_host + static_cast<size_t>(s->N) * s->Q);
   else s->last_set.clear();
@@ -186,6 +220,7 @@ int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
   if (nsteps < 0) return fail(LBM_ERR_ARG, "nsteps < 0");
   HIP_TRY(hipSetDevice(s->prm.device));
   const bool force = has_force(s);
+  HIP_TRY(hipEventRecord(s->ev0, s->stream));
   for (int32_t t = 0; t < nsteps; ++t) {
     StepArgs a = args_for(s, s->cur);
     if (coef) {
@@ -197,13 +232,27 @@ int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
       a.coef[0] = U;
     }
     a.wss_on = (s->prm.wss_every > 0 && (s->iter % s->prm.wss_every) == 0) ? 1 : 0;
-    HIP_TRY(launch_lbm(s->Q, s->prec, KMODE_STEP, force, a, s->stream));
+    if (s->family == 1) {
+      ColArgs c = col_args_for(s, s->cur);
+      std::memcpy(c.coef, a.coef, sizeof(c.coef));
+      HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->stream));
+      if (a.wss_on) {
+        ColWssArgs w = s->wss_args;
+        w.fs_in = s->fs[s->cur];
+        w.porosity = s->porosity;
+        HIP_TRY(launch_columns_wss(s->Q, s->prec, force, w, s->stream));
+      }
+    } else {
+      HIP_TRY(launch_lbm(s->Q, s->prec, KMODE_STEP, force, a, s->stream));
+    }
     std::memcpy(s->last_coef, a.coef, sizeof(s->last_coef));
     s->cur = 1 - s->cur;
     s->iter++;
     if (s->prm.flow_type == LBM_FLOW_PULSATILE) s->total_runtime += s->prm.dt;
     s->stepped = true;
   }
+  HIP_TRY(hipEventRecord(s->ev1, s->stream));
+  s->batch_steps = nsteps;
   if (s->base.zero_check && nsteps > 0) {
     uint32_t flags = 0;
     HIP_TRY(hipMemcpyAsync(&flags, s->err, sizeof(flags), hipMemcpyDeviceToHost, s->stream));
@@ -215,6 +264,220 @@ int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
                   "replay would need its previous populations");
     }
   }
+  return LBM_OK;
+}
+
+void xcd_tables(int32_t Bs, int32_t Bb, int32_t* first, int32_t* offset) {
+  int32_t off = 0;
+  for (int x = 0; x < 8; ++x) {
+    const int32_t f = Bs + ((x - Bs % 8) + 8) % 8;
+    const int32_t cnt = f < Bs + Bb ? (Bs + Bb - 1 - f) / 8 + 1 : 0;
+    first[x] = f;
+    offset[x] = off;
+    off += cnt;
+  }
+}
+
+// Layout of the column-offset family (columns.h): slots = reference node
+// order; regular nodes store 8 lateral column slots as 16-bit offsets from a
+// per-64-slot base, irregular nodes an explicit int32 pull row.
+int create_columns(lbm_solver* s, const Tables& t) {
+  const int Q = t.Q;
+  const int64_t N = t.N;
+  const lbm::LatticeView L = lbm::lattice_view(Q);
+  const lbm_params& p = s->prm;
+  s->npad = std::max<int64_t>(round_up(N, BLOCK), BLOCK);
+  if (s->npad >= (int64_t(1) << 31)) return fail(LBM_ERR_ARG, "too many nodes for int32 slots");
+  const int64_t npad = s->npad;
+  s->slot_of.resize(N);
+  s->node_of.assign(npad, -1);
+  for (int64_t n = 0; n < N; ++n) {
+    s->slot_of[n] = static_cast<int32_t>(n);
+    s->node_of[n] = static_cast<int32_t>(n);
+  }
+  s->n_special = 0;
+  s->n_special_pad = 0;
+  s->n_bulk = N;
+  s->bulk_end = N;
+  s->nw_slots = t.nearwall;
+
+  auto srcq = [&](int q, int64_t n) -> int32_t {
+    return q == L.rest ? static_cast<int32_t>(n) : t.nb[static_cast<size_t>(L.opp[q]) * N + n];
+  };
+  int qcol[NCOL];
+  for (int o = 0; o < NCOL; ++o) {
+    qcol[o] = -1;
+    for (int q = 0; q < Q; ++q)
+      if (L.cz[q] == 0 && col_of(L.cx[q], L.cy[q]) == o) qcol[o] = q;
+    if (qcol[o] < 0) return fail(LBM_ERR_ARG, "lattice lacks an in-plane direction");
+  }
+  const bool inl_b = p.inlet_mode != LBM_INLET_NONE, out_b = p.outlet_mode != LBM_OUTLET_NONE;
+  std::vector<uint8_t> reg(N, 0);
+  std::vector<int32_t> col(static_cast<size_t>(NCOL) * N, -1);
+  for (int64_t n = 0; n < N; ++n) {
+    for (int o = 0; o < NCOL; ++o) col[static_cast<size_t>(o) * N + n] = srcq(qcol[o], n);
+    bool r = !((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b));
+    for (int q = 0; q < Q && r; ++q) {
+      const int32_t sq = srcq(q, n);
+      const int o = col_of(L.cx[q], L.cy[q]);
+      const int64_t base = o < 0 ? n : col[static_cast<size_t>(o) * N + n];
+      if (sq < 0 || base < 0 || int64_t(sq) != base - L.cz[q]) r = false;
+    }
+    reg[n] = r ? 1 : 0;
+  }
+  const int64_t nchunk = npad / CHUNK;
+  std::vector<int32_t> cb(static_cast<size_t>(NCOL) * nchunk, 0);
+  std::vector<uint16_t> idx(static_cast<size_t>(NCOL) * npad, 0);
+  for (int64_t c = 0; c < nchunk; ++c) {
+    const int64_t a0 = c * CHUNK, a1 = std::min<int64_t>(N, a0 + CHUNK);
+    for (bool again = true; again;) {
+      again = false;
+      for (int o = 0; o < NCOL && !again; ++o) {
+        int64_t mn = INT64_MAX;
+        for (int64_t n = a0; n < a1; ++n)
+          if (reg[n]) mn = std::min<int64_t>(mn, col[static_cast<size_t>(o) * N + n]);
+        if (mn == INT64_MAX) mn = 0;
+        for (int64_t n = a0; n < a1; ++n)
+          if (reg[n] && col[static_cast<size_t>(o) * N + n] - mn > 0xFFFE) {
+            reg[n] = 0;  // too far from its chunk: give it an explicit row
+            again = true;
+          }
+        cb[static_cast<size_t>(o) * nchunk + c] = static_cast<int32_t>(mn);
+      }
+    }
+    for (int o = 0; o < NCOL; ++o)
+      for (int64_t n = a0; n < a1; ++n)
+        if (reg[n])
+          idx[static_cast<size_t>(o) * npad + n] =
+              static_cast<uint16_t>(col[static_cast<size_t>(o) * N + n] - cb[static_cast<size_t>(o) * nchunk + c]);
+  }
+  std::vector<int32_t> inlet_index(N, -1);
+  for (size_t i = 0; i < t.inlet_nodes.size(); ++i) inlet_index[t.inlet_nodes[i]] = static_cast<int32_t>(i);
+  std::vector<int32_t> rows;
+  for (int64_t n = 0; n < N; ++n)
+    if (!reg[n]) rows.push_back(static_cast<int32_t>(n));
+  const int64_t nirr = static_cast<int64_t>(rows.size());
+  s->n_irr = nirr;
+  std::vector<int32_t> isrc(static_cast<size_t>(Q) * std::max<int64_t>(nirr, 1), -1);
+  std::vector<uint32_t> ikind(std::max<int64_t>(nirr, 1), 0);
+  std::vector<int32_t> ibcn(std::max<int64_t>(nirr, 1), -1), iinl(std::max<int64_t>(nirr, 1), -1);
+  std::vector<int32_t> bsl;
+  for (int64_t r = 0; r < nirr; ++r) {
+    const int32_t n = rows[r];
+    for (int q = 0; q < Q; ++q) isrc[static_cast<size_t>(q) * nirr + r] = srcq(q, n);
+    ikind[r] = t.kind[n];
+    ibcn[r] = t.bc_nbr[n];
+    iinl[r] = inlet_index[n];
+    idx[n] = IRREGULAR;
+    idx[static_cast<size_t>(npad) + n] = static_cast<uint16_t>(r & 0xFFFF);
+    idx[static_cast<size_t>(2) * npad + n] = static_cast<uint16_t>(uint32_t(r) >> 16);
+    if ((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b)) bsl.push_back(n);
+  }
+  // Decode every regular slot the way the kernel does and compare with the
+  // int32 pull table.
+  for (int64_t n = 0; n < N; ++n) {
+    if (!reg[n]) continue;
+    const int64_t c = n / CHUNK;
+    for (int q = 0; q < Q; ++q) {
+      const int o = col_of(L.cx[q], L.cy[q]);
+      const int64_t colv = o < 0 ? n
+                                 : int64_t(cb[static_cast<size_t>(o) * nchunk + c]) +
+                                       idx[static_cast<size_t>(o) * npad + n];
+      if (colv - L.cz[q] != srcq(q, n))
+        return fail(LBM_ERR_STATE, "column-offset encoding does not reproduce the neighbour table");
+    }
+    if (idx[n] == IRREGULAR) return fail(LBM_ERR_STATE, "regular slot carries the irregular marker");
+  }
+  const int64_t nnw = static_cast<int64_t>(t.nearwall.size());
+  std::vector<int32_t> nwsrc(static_cast<size_t>(Q) * std::max<int64_t>(nnw, 1), -1);
+  for (int64_t i = 0; i < nnw; ++i)
+    for (int q = 0; q < Q; ++q) nwsrc[static_cast<size_t>(q) * nnw + i] = srcq(q, t.nearwall[i]);
+
+  int rc = LBM_OK;
+  const size_t pop_bytes = static_cast<size_t>(Q) * npad * s->esize;
+  const size_t ninl = std::max<size_t>(t.inlet_nodes.size(), 1);
+  const int64_t nblk = npad / BLOCK;
+  if ((rc = dalloc_raw(s, &s->fs[0], pop_bytes)) || (rc = dalloc_raw(s, &s->fs[1], pop_bytes)) ||
+      (rc = dalloc(s, &s->idx16, idx.size())) || (rc = dalloc(s, &s->cbase, cb.size())) ||
+      (rc = dalloc(s, &s->irr_src, isrc.size())) || (rc = dalloc(s, &s->irr_kind, ikind.size())) ||
+      (rc = dalloc(s, &s->irr_bcn, ibcn.size())) || (rc = dalloc(s, &s->irr_inlet, iinl.size())) ||
+      (rc = dalloc(s, &s->bslots, std::max<size_t>(bsl.size(), 1))) ||
+      (rc = dalloc(s, &s->nw_slot, std::max<int64_t>(nnw, 1))) || (rc = dalloc(s, &s->nw_src, nwsrc.size())) ||
+      (rc = dalloc(s, &s->nw_normal, 3 * std::max<int64_t>(nnw, 1))) ||
+      (rc = dalloc(s, &s->nw_wss, 5 * std::max<int64_t>(nnw, 1))) || (rc = dalloc(s, &s->err, 1)) ||
+      (rc = dalloc(s, &s->inlet_basis, MAX_MODES * ninl)) ||
+      (rc = dalloc(s, &s->partials, static_cast<size_t>(4 * nblk + 2 * ((nnw + BLOCK - 1) / BLOCK) + 8))) ||
+      (rc = dalloc(s, &s->red_out, 8)))
+    return rc;
+  std::vector<double> basis(MAX_MODES * ninl, 0.0);
+  for (size_t i = 0; i < ninl; ++i) basis[i] = 1.0;
+  auto up = [&](void* d, const void* h, size_t bytes) {
+    return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s->stream) : hipSuccess;
+  };
+  if (up(s->idx16, idx.data(), idx.size() * 2) || up(s->cbase, cb.data(), cb.size() * 4) ||
+      up(s->irr_src, isrc.data(), isrc.size() * 4) || up(s->irr_kind, ikind.data(), ikind.size() * 4) ||
+      up(s->irr_bcn, ibcn.data(), ibcn.size() * 4) || up(s->irr_inlet, iinl.data(), iinl.size() * 4) ||
+      up(s->bslots, bsl.data(), bsl.size() * 4) || up(s->nw_slot, t.nearwall.data(), nnw * 4) ||
+      up(s->nw_src, nwsrc.data(), nwsrc.size() * 4) || up(s->nw_normal, t.normals.data(), 3 * nnw * 8) ||
+      up(s->inlet_basis, basis.data(), basis.size() * 8) ||
+      hipMemsetAsync(s->nw_wss, 0, 5 * std::max<int64_t>(nnw, 1) * 8, s->stream) ||
+      hipMemsetAsync(s->err, 0, 4, s->stream) || hipMemsetAsync(s->fs[0], 0, pop_bytes, s->stream) ||
+      hipMemsetAsync(s->fs[1], 0, pop_bytes, s->stream))
+    return fail(LBM_ERR_HIP, "upload failed");
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(LBM_ERR_HIP, "upload sync failed");
+
+  const double omega = 1.0 / p.tau;
+  const double spacing = p.spacing > 0 ? p.spacing : t.grid.spacing;
+  const double wss_C = 1060.0 * (p.dt * p.dt) / (spacing * spacing * 1e-6);
+
+  ColArgs& c = s->cbase_args;
+  std::memset(&c, 0, sizeof(c));
+  c.npad = npad;
+  c.n = static_cast<int32_t>(N);
+  c.nblocks = static_cast<int32_t>(nblk);
+  c.xcd_remap = 1;
+  xcd_tables(0, c.nblocks, c.xcd_first, c.xcd_offset);
+  c.idx16 = s->idx16;
+  c.cbase = s->cbase;
+  c.nchunk = static_cast<int32_t>(nchunk);
+  c.n_irr = static_cast<int32_t>(nirr);
+  c.irr_src = s->irr_src;
+  c.irr_kind = s->irr_kind;
+  c.irr_bcn = s->irr_bcn;
+  c.irr_inlet = s->irr_inlet;
+  c.bslots = s->bslots;
+  c.n_bslots = static_cast<int32_t>(bsl.size());
+  c.n_inlet = static_cast<int32_t>(ninl);
+  c.n_modes = 1;
+  c.inlet_basis = s->inlet_basis;
+  for (int i = 0; i < 3; ++i) c.inlet_normal[i] = t.inlet_normal[i];
+  c.inlet_mode = p.inlet_mode;
+  c.outlet_mode = p.outlet_mode;
+  c.outlet_rho = p.outlet_rho;
+  c.omega = omega;
+  for (int i = 0; i < 3; ++i) c.force[i] = p.force[i];
+
+  ColWssArgs& w = s->wss_args;
+  std::memset(&w, 0, sizeof(w));
+  w.npad = npad;
+  w.n_nw = static_cast<int32_t>(nnw);
+  w.slot = s->nw_slot;
+  w.src = s->nw_src;
+  w.normal = s->nw_normal;
+  w.wss = s->nw_wss;
+  w.wss_C = wss_C;
+  w.omega = omega;
+  for (int i = 0; i < 3; ++i) w.force[i] = p.force[i];
+
+  // Generic fields used by the shared moments kernel and init_state.
+  StepArgs& a = s->base;
+  std::memset(&a, 0, sizeof(a));
+  a.npad = npad;
+  a.n_special = 0;
+  a.n_special_pad = 0;
+  a.bulk_end = static_cast<int32_t>(N);
+  a.omega = omega;
+  a.err_flags = s->err;
   return LBM_OK;
 }
 
@@ -280,6 +543,16 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
     return bail(fail(LBM_ERR_HIP, "stream/event creation failed"));
+
+  if (p->wall_mode == LBM_WALL_HALFWAY_BB) {
+    s->family = 1;
+    int rc = create_columns(s, t);
+    if (rc == LBM_OK) rc = init_state(s, nullptr);
+    if (rc == LBM_OK && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(LBM_ERR_HIP, "init failed");
+    if (rc) return bail(rc);
+    *out = s;
+    return LBM_OK;
+  }
 
   // Slot layout: special nodes (wall, inlet, outlet, near-wall) first, then
   // plain fluid, each in reference order.
@@ -470,15 +743,20 @@ int lbm_step_coefficients(lbm_solver* s, int32_t nsteps, const double* coef) {
 
 int lbm_step_timed(lbm_solver* s, int32_t nsteps, double* ms) {
   if (!s || !ms) return fail(LBM_ERR_ARG, "null argument");
-  HIP_TRY(hipSetDevice(s->prm.device));
-  HIP_TRY(hipEventRecord(s->ev0, s->stream));
   int rc = run_steps(s, nsteps, nullptr);
   if (rc) return rc;
-  HIP_TRY(hipEventRecord(s->ev1, s->stream));
+  int32_t n = 0;
+  return lbm_last_batch_time(s, ms, &n);
+}
+
+int lbm_last_batch_time(lbm_solver* s, double* ms, int32_t* nsteps) {
+  if (!s || !ms) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
   HIP_TRY(hipEventSynchronize(s->ev1));
   float e = 0.0f;
   HIP_TRY(hipEventElapsedTime(&e, s->ev0, s->ev1));
   *ms = e;
+  if (nsteps) *nsteps = s->batch_steps;
   return LBM_OK;
 }
 
@@ -512,7 +790,14 @@ int lbm_get_populations(lbm_solver* s, double* f) {
   std::memcpy(a.coef, s->last_coef, sizeof(a.coef));
   a.f_out = s->f_tmp;
   a.wss_on = 0;
-  HIP_TRY(launch_lbm(Q, s->prec, KMODE_MATERIALIZE, false, a, s->stream));
+  if (s->family == 1) {
+    ColArgs c = col_args_for(s, 1 - s->cur);
+    std::memcpy(c.coef, s->last_coef, sizeof(c.coef));
+    c.f_out = s->f_tmp;
+    HIP_TRY(launch_columns(Q, s->prec, KMODE_MATERIALIZE, false, c, s->stream));
+  } else {
+    HIP_TRY(launch_lbm(Q, s->prec, KMODE_MATERIALIZE, false, a, s->stream));
+  }
   std::vector<double> soa(static_cast<size_t>(Q) * s->npad);
   HIP_TRY(hipMemcpyAsync(soa.data(), s->f_tmp, soa.size() * 8, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -557,6 +842,21 @@ int lbm_get_post_collision(lbm_solver* s, double* fs) {
 int lbm_get_wss(lbm_solver* s, double* wss, double* acc_tau, double* acc_mag) {
   if (!s) return fail(LBM_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->prm.device));
+  if (s->family == 1) {
+    const int64_t m = static_cast<int64_t>(s->nw_slots.size());
+    std::vector<double> h(static_cast<size_t>(5 * m));
+    if (m) {
+      HIP_TRY(hipMemcpyAsync(h.data(), s->nw_wss, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+      HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    for (int64_t i = 0; i < m; ++i) {
+      if (wss) wss[i] = h[i];
+      if (acc_tau)
+        for (int a = 0; a < 3; ++a) acc_tau[3 * i + a] = h[(1 + a) * m + i];
+      if (acc_mag) acc_mag[i] = h[4 * m + i];
+    }
+    return LBM_OK;
+  }
   const int64_t sp = s->n_special_pad;
   std::vector<double> h(static_cast<size_t>(5 * sp));
   if (sp) {
@@ -623,7 +923,10 @@ int lbm_reduce(lbm_solver* s, lbm_stats* out) {
   HIP_TRY(hipSetDevice(s->prm.device));
   StepArgs a = args_for(s, s->cur);
   HIP_TRY(launch_moments(s->Q, s->prec, a, s->partials, s->red_out, s->stream));
-  HIP_TRY(launch_wss_stats(a, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
+  if (s->family == 1)
+    HIP_TRY(launch_columns_wss_stats(s->wss_args, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
+  else
+    HIP_TRY(launch_wss_stats(a, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
   double h[6];
   HIP_TRY(hipMemcpyAsync(h, s->red_out, sizeof(h), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));

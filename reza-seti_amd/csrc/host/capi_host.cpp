@@ -52,6 +52,7 @@ static BuildOptions to_opts(const lbm_geometry_options* o) {
   b.nthreads = o->nthreads;
   b.verify_stride = o->verify_stride;
   b.compute_normals = o->compute_normals;
+  b.device = o->device;
   return b;
 }
 

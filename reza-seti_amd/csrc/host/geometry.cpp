@@ -14,6 +14,7 @@
 #include <unordered_set>
 
 #include "../lattice.h"
+#include "normals.h"
 
 namespace lbm {
 
@@ -284,32 +285,42 @@ bool label_points(const GridSpec& g, const std::vector<Triangle>& tris, std::vec
   return true;
 }
 
-std::array<double, 3> boundary_normal(double x, double y, double z, const std::vector<Triangle>& tris) {
-  // SubDomain::calculateBoundaryNormals body (Subdomain.cpp:519-570). The
-  // reference's std::pow(d, 2) is the exact square (compilers fold it to d*d).
+// Per-triangle part of SubDomain::calculateBoundaryNormals (Subdomain.cpp:530-550).
+static TriTerms tri_terms(const Triangle& T) {
+  const double ux = T.x2 - T.x1, uy = T.y2 - T.y1, uz = T.z2 - T.z1;
+  const double vx = T.x3 - T.x1, vy = T.y3 - T.y1, vz = T.z3 - T.z1;
+  double nx = uy * vz - uz * vy;
+  double ny = uz * vx - ux * vz;
+  double nz = ux * vy - uy * vx;
+  const double norm = std::sqrt(nx * nx + ny * ny + nz * nz);
+  if (norm > 0) {
+    nx /= norm;
+    ny /= norm;
+    nz /= norm;
+  }
+  TriTerms t;
+  t.nx = nx;
+  t.ny = ny;
+  t.nz = nz;
+  t.area = 0.5 * norm;
+  t.cx = (T.x1 + T.x2 + T.x3) / 3.0;
+  t.cy = (T.y1 + T.y2 + T.y3) / 3.0;
+  t.cz = (T.z1 + T.z2 + T.z3) / 3.0;
+  t.pad = 0.0;
+  return t;
+}
+
+// Point part (Subdomain.cpp:551-570). The reference's std::pow(d, 2) is the
+// exact square (compilers fold it to d*d).
+static std::array<double, 3> normal_from_terms(double x, double y, double z, const std::vector<TriTerms>& tt) {
   double n0 = 0.0, n1 = 0.0, n2 = 0.0;
-  for (const Triangle& T : tris) {
-    const double ux = T.x2 - T.x1, uy = T.y2 - T.y1, uz = T.z2 - T.z1;
-    const double vx = T.x3 - T.x1, vy = T.y3 - T.y1, vz = T.z3 - T.z1;
-    double nx = uy * vz - uz * vy;
-    double ny = uz * vx - ux * vz;
-    double nz = ux * vy - uy * vx;
-    const double norm = std::sqrt(nx * nx + ny * ny + nz * nz);
-    if (norm > 0) {
-      nx /= norm;
-      ny /= norm;
-      nz /= norm;
-    }
-    const double area = 0.5 * norm;
-    const double cX = (T.x1 + T.x2 + T.x3) / 3.0;
-    const double cY = (T.y1 + T.y2 + T.y3) / 3.0;
-    const double cZ = (T.z1 + T.z2 + T.z3) / 3.0;
-    const double ddx = x - cX, ddy = y - cY, ddz = z - cZ;
+  for (const TriTerms& T : tt) {
+    const double ddx = x - T.cx, ddy = y - T.cy, ddz = z - T.cz;
     const double distance = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-    const double weight = area / (distance * distance);
-    n0 += nx * weight;
-    n1 += ny * weight;
-    n2 += nz * weight;
+    const double weight = T.area / (distance * distance);
+    n0 += T.nx * weight;
+    n1 += T.ny * weight;
+    n2 += T.nz * weight;
   }
   const double norm = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
   if (norm > 0) {
@@ -318,6 +329,13 @@ std::array<double, 3> boundary_normal(double x, double y, double z, const std::v
     n2 /= norm;
   }
   return {n0, n1, n2};
+}
+
+std::array<double, 3> boundary_normal(double x, double y, double z, const std::vector<Triangle>& tris) {
+  std::vector<TriTerms> tt;
+  tt.reserve(tris.size());
+  for (const Triangle& T : tris) tt.push_back(tri_terms(T));
+  return normal_from_terms(x, y, z, tt);
 }
 
 namespace {
@@ -526,13 +544,29 @@ bool build_from_labels(const GridSpec& g, const std::vector<uint8_t>& fluid,
   }
   t.normals.assign(3 * t.nearwall.size(), 0.0);
   if (o.compute_normals) {
-    parallel_for(static_cast<int64_t>(t.nearwall.size()), o.nthreads, [&](int64_t a) {
+    std::vector<TriTerms> tt;
+    tt.reserve(tris.size());
+    for (const Triangle& T : tris) tt.push_back(tri_terms(T));
+    std::vector<double> pts(3 * t.nearwall.size());
+    for (size_t a = 0; a < t.nearwall.size(); ++a) {
       const int32_t n = t.nearwall[a];
-      const auto nv = boundary_normal(g.px(t.ijk[3 * n]), g.py(t.ijk[3 * n + 1]), g.pz(t.ijk[3 * n + 2]), tris);
-      t.normals[3 * a] = nv[0];
-      t.normals[3 * a + 1] = nv[1];
-      t.normals[3 * a + 2] = nv[2];
-    });
+      pts[3 * a] = g.px(t.ijk[3 * n]);
+      pts[3 * a + 1] = g.py(t.ijk[3 * n + 1]);
+      pts[3 * a + 2] = g.pz(t.ijk[3 * n + 2]);
+    }
+    if (o.compute_normals == 2) {
+      // Same sum evaluated on the GPU (bit identical; see normals.hip).
+      std::vector<double> out;
+      if (!device_boundary_normals(pts, tt, out, o.device, err)) return false;
+      t.normals.swap(out);
+    } else {
+      parallel_for(static_cast<int64_t>(t.nearwall.size()), o.nthreads, [&](int64_t a) {
+        const auto nv = normal_from_terms(pts[3 * a], pts[3 * a + 1], pts[3 * a + 2], tt);
+        t.normals[3 * a] = nv[0];
+        t.normals[3 * a + 1] = nv[1];
+        t.normals[3 * a + 2] = nv[2];
+      });
+    }
   }
   return true;
 }

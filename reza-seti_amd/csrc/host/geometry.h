@@ -60,7 +60,8 @@ struct BuildOptions {
   int periodic[3] = {0, 0, 0};  // extension: wrap neighbours along an axis
   int nthreads = 0;              // 0 = hardware concurrency
   int verify_stride = 0;         // see label_points
-  int compute_normals = 1;
+  int compute_normals = 1;       // 0 none, 1 host threads, 2 GPU (same sum)
+  int device = 0;
 };
 
 struct Tables {

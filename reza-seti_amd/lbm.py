@@ -48,7 +48,7 @@ class Config(C.Structure):
 
 class GeometryOptions(C.Structure):
     _fields_ = [("q", C.c_int32), ("periodic", C.c_int32 * 3), ("nthreads", C.c_int32),
-                ("verify_stride", C.c_int32), ("compute_normals", C.c_int32)]
+                ("verify_stride", C.c_int32), ("compute_normals", C.c_int32), ("device", C.c_int32)]
 
 
 class GeometryInfo(C.Structure):
@@ -113,6 +113,7 @@ SIGNATURES = [
     ("lbm_get_macroscopic", [_P, _P, _P], C.c_int),
     ("lbm_reduce", [_P, _P], C.c_int),
     ("lbm_step_timed", [_P, C.c_int32, _P], C.c_int),
+    ("lbm_last_batch_time", [_P, _P, _P], C.c_int),
     ("lbm_synchronize", [_P], C.c_int),
     ("lbm_destroy", [_P], None),
     ("lbm_get_solver_info", [_P, _P], C.c_int),
@@ -190,7 +191,7 @@ class Geometry:
             o.periodic[a] = int(periodic[a])
         o.nthreads = nthreads
         o.verify_stride = verify_stride
-        o.compute_normals = 1 if normals else 0
+        o.compute_normals = int(normals)
         return o
 
     @classmethod
@@ -310,6 +311,12 @@ class Solver:
         ms = C.c_double()
         _check(lib().lbm_step_timed(self._h, int(nsteps), C.byref(ms)))
         return ms.value
+
+    def last_batch_ms(self):
+        ms = C.c_double()
+        n = C.c_int32()
+        _check(lib().lbm_last_batch_time(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def set_inlet_profile(self, basis):
         basis = np.ascontiguousarray(np.atleast_2d(basis), dtype=np.float64)

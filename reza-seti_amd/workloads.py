@@ -162,3 +162,185 @@ def icosphere(center, radius, subdiv=2, rotation_seed=None, radial=None):
     tris = pts[np.array(f)]
     n = _unit(np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0]))
     return tris, n
+
+
+# ----------------------------------------------------------------- meshing
+# Kuhn split of a cube (corner index = x + 2y + 4z) into six tetrahedra that
+# share the 0-7 diagonal; neighbouring cubes split shared faces the same way,
+# so the extracted surface is closed.
+_KUHN = np.array([[0, 1, 3, 7], [0, 3, 2, 7], [0, 2, 6, 7], [0, 6, 4, 7], [0, 4, 5, 7], [0, 5, 1, 7]])
+
+
+def marching_tetrahedra(func, lo, hi, h, offset=(0.2113, 0.3571, 0.1237), chunk=48):
+    """Zero level set of func(x, y, z) (negative inside) on a grid of pitch h
+    covering [lo, hi]. Edge points are interpolated from the endpoint with the
+    smaller global index, so a shared edge gives bit-identical vertices in
+    every tetrahedron that meets it. Returns (tris[T,3,3], normals[T,3])."""
+    lo = np.asarray(lo, np.float64) - h + np.asarray(offset) * h
+    n = np.ceil((np.asarray(hi, np.float64) + h - lo) / h).astype(int) + 1
+    nx, ny, nz = (int(v) for v in n)
+    xs = lo[0] + np.arange(nx) * h
+    ys = lo[1] + np.arange(ny) * h
+    zs = lo[2] + np.arange(nz) * h
+    out_t, out_n = [], []
+    for i0 in range(0, nx - 1, chunk):
+        i1 = min(nx, i0 + chunk + 1)
+        X, Y, Z = np.meshgrid(xs[i0:i1], ys, zs, indexing="ij")
+        F = func(X, Y, Z)
+        F[F == 0.0] = 1e-12
+        pos = F > 0
+        cx, cy, cz = F.shape[0] - 1, ny - 1, nz - 1
+        corners = [pos[a:a + cx, b:b + cy, c:c + cz] for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+        anyp = np.zeros_like(corners[0])
+        allp = np.ones_like(corners[0])
+        for cc in corners:
+            anyp |= cc
+            allp &= cc
+        act = np.argwhere(anyp & ~allp)
+        if len(act) == 0:
+            continue
+        off = np.array([[a, b, c] for c in (0, 1) for b in (0, 1) for a in (0, 1)])
+        for tet in _KUHN:
+            vi = act[:, None, :] + off[tet][None, :, :]          # local (i, j, k) of the 4 corners
+            gi = (vi[..., 0] + i0) * (ny * nz) + vi[..., 1] * nz + vi[..., 2]
+            val = F[vi[..., 0], vi[..., 1], vi[..., 2]]
+            pt = np.stack([xs[vi[..., 0] + i0], ys[vi[..., 1]], zs[vi[..., 2]]], -1)
+            sp = val > 0
+            npos = sp.sum(1)
+
+            def cut(a, b, sel):
+                ga, gb = gi[sel, a], gi[sel, b]
+                sw = ga > gb
+                A = np.where(sw[:, None], pt[sel, b], pt[sel, a])
+                B = np.where(sw[:, None], pt[sel, a], pt[sel, b])
+                fa = np.where(sw, val[sel, b], val[sel, a])
+                fb = np.where(sw, val[sel, a], val[sel, b])
+                t = fa / (fa - fb)
+                return A + t[:, None] * (B - A)
+
+            grad_sel = []
+            for k in (1, 3):
+                sel = np.where(npos == k)[0]
+                if len(sel) == 0:
+                    continue
+                lone_is_pos = (k == 1)
+                order = np.argsort(~sp[sel] if lone_is_pos else sp[sel], axis=1, kind="stable")
+                L, o1, o2, o3 = (order[:, m] for m in range(4))
+                p1 = _cut_idx(cut, sel, L, o1)
+                p2 = _cut_idx(cut, sel, L, o2)
+                p3 = _cut_idx(cut, sel, L, o3)
+                grad_sel.append((np.stack([p1, p2, p3], 1), _grad(pt[sel], sp[sel])))
+            sel = np.where(npos == 2)[0]
+            if len(sel):
+                order = np.argsort(~sp[sel], axis=1, kind="stable")
+                P1, P2, N1, N2 = (order[:, m] for m in range(4))
+                a = _cut_idx(cut, sel, P1, N1)
+                b = _cut_idx(cut, sel, P1, N2)
+                c = _cut_idx(cut, sel, P2, N2)
+                d = _cut_idx(cut, sel, P2, N1)
+                g = _grad(pt[sel], sp[sel])
+                grad_sel.append((np.stack([a, b, c], 1), g))
+                grad_sel.append((np.stack([a, c, d], 1), g))
+            for tri, g in grad_sel:
+                nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+                flip = (nrm * g).sum(1) < 0
+                tri[flip] = tri[flip][:, [0, 2, 1]]
+                nrm[flip] = -nrm[flip]
+                ln = np.linalg.norm(nrm, axis=1)
+                keep = ln > 1e-12
+                out_t.append(tri[keep])
+                out_n.append(nrm[keep] / ln[keep, None])
+    return np.concatenate(out_t), np.concatenate(out_n)
+
+
+def _cut_idx(cut, sel, a, b):
+    # per-row endpoints a[i], b[i] (tetra-local corner numbers)
+    out = np.empty((len(sel), 3))
+    for ca in range(4):
+        for cb in range(4):
+            m = (a == ca) & (b == cb)
+            if m.any():
+                out[m] = cut(ca, cb, sel[m])
+    return out
+
+
+def _grad(pt, sp):
+    w = sp.astype(np.float64)
+    npos = w.sum(1, keepdims=True)
+    pc = (pt * w[..., None]).sum(1) / npos
+    nc = (pt * (1 - w)[..., None]).sum(1) / (4 - npos)
+    return pc - nc
+
+
+def _segment_distance(X, Y, Z, p0, p1):
+    d = np.asarray(p1, np.float64) - np.asarray(p0, np.float64)
+    L = np.linalg.norm(d)
+    d = d / L
+    rx, ry, rz = X - p0[0], Y - p0[1], Z - p0[2]
+    s = rx * d[0] + ry * d[1] + rz * d[2]
+    sc = np.clip(s, 0.0, L)
+    qx, qy, qz = rx - sc * d[0], ry - sc * d[1], rz - sc * d[2]
+    return np.sqrt(qx * qx + qy * qy + qz * qz), s, (qx, qy, qz), d
+
+
+def bifurcation_function(parent_radius=24.0, daughter_radius=18.0, half_angle_deg=35.0, parent_length=600.0,
+                         daughter_length=900.0, amplitude=2.0, seed=1):
+    """Implicit Y-bifurcation (negative inside): parent along +z from z = 0,
+    two daughters at +-half_angle in the xz plane, cut by z = 0 and by a
+    horizontal plane through the daughters; radii carry a seeded
+    three-mode sinusoidal perturbation with total amplitude <= `amplitude`."""
+    rng = np.random.default_rng(seed)
+    amps = rng.uniform(0.2, 1.0, 3)
+    amps *= amplitude / amps.sum()
+    kz = rng.uniform(2 * np.pi / 400, 2 * np.pi / 80, 3)
+    nth = rng.integers(0, 4, 3)
+    ph = rng.uniform(0, 2 * np.pi, 3)
+    a = np.deg2rad(half_angle_deg)
+    P0 = np.array([0.0, 0.0, -20.0])
+    P1 = np.array([0.0, 0.0, parent_length])
+    D = [P1 + daughter_length * np.array([np.sin(a), 0.0, np.cos(a)]),
+         P1 + daughter_length * np.array([-np.sin(a), 0.0, np.cos(a)])]
+    z_top = parent_length + (daughter_length - daughter_radius - 4.0) * np.cos(a)
+
+    def radius(R0, s, q):
+        th = np.arctan2(q[1], q[0])
+        r = np.full_like(s, R0)
+        for m in range(3):
+            r = r + amps[m] * np.sin(kz[m] * s + nth[m] * th + ph[m])
+        return r
+
+    def F(X, Y, Z):
+        dist, s, q, _ = _segment_distance(X, Y, Z, P0, P1)
+        f = dist - radius(parent_radius, s, q)
+        for Pd in D:
+            dd, sd, qd, _ = _segment_distance(X, Y, Z, P1, Pd)
+            f = np.minimum(f, dd - radius(daughter_radius, sd + parent_length, qd))
+        f = np.maximum(f, -Z)
+        f = np.maximum(f, Z - z_top)
+        return f
+
+    ext = daughter_length * np.sin(a) + daughter_radius + amplitude + 2
+    lo = (-ext, -(parent_radius + amplitude + 2), 0.0)
+    hi = (ext, parent_radius + amplitude + 2, z_top)
+    return F, lo, hi, dict(z_top=z_top, parent_axis=(0.0, 0.0), parent_radius=parent_radius)
+
+
+def c1_bifurcation(workdir, mesh_h=3.5, **kw):
+    """Y-bifurcation of BASELINE config 1 (parent r 24, daughters r 18 at 35
+    degrees, 3-mode wall perturbation of amplitude 2, seed 1). Inlet: the z = 0
+    cap (normal -z); outlet: the top cut plane (normal +z), set on the last
+    grid layer inside the cap. Returns (config path, info dict)."""
+    os.makedirs(workdir, exist_ok=True)
+    F, lo, hi, info = bifurcation_function(**kw)
+    tris, nrm = marching_tetrahedra(F, lo, hi, mesh_h)
+    stl = os.path.join(workdir, "bifurcation.stl")
+    write_stl(stl, tris, nrm, "bifurcation")
+    zmin, zmax = float(tris[..., 2].min()), float(tris[..., 2].max())
+    k_top = math.floor((zmax - zmin) / 1.0)
+    z_out = zmin + k_top * 1.0
+    cfg = os.path.join(workdir, "config.txt")
+    write_config(cfg, "bifurcation.stl", tau=0.67, max_iter=8000,
+                 inlets=[dict(plane="z", coordinate=zmin, normal=(0, 0, -1))],
+                 outlets=[dict(plane="z", coordinate=z_out, normal=(0, 0, 1))])
+    info.update(n_triangles=len(tris), z_in=zmin, z_out=z_out)
+    return cfg, info

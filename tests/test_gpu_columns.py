@@ -1,0 +1,111 @@
+"""Column-offset kernel family (half-way bounce-back walls) on the GPU.
+
+These boundary modes have no reference counterpart; they are checked against
+the oracle's extension modes (same algorithm, push form, FP64) and against
+closed forms: periodic Poiseuille flow under a body force, mass conservation.
+The FP64 GPU path forms FMAs in the equilibrium, so agreement with the oracle
+is to rounding (1e-12 on populations of order w_q), not bitwise.
+"""
+import numpy as np
+import pytest
+
+import lbm
+import oracle
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+TAU, DT, U = 0.8, 1e-5, 0.04
+
+
+@pytest.mark.parametrize("inlet,outlet", [(lbm.INLET_NEE_VELOCITY, lbm.OUTLET_NEE_PRESSURE),
+                                          (lbm.INLET_REFERENCE_ZOUHE, lbm.OUTLET_REFERENCE)])
+def test_bounce_back_family_matches_oracle(c0_geometry, inlet, outlet):
+    g = c0_geometry
+    sol = lbm.Solver(g, tau=TAU, dt=DT, cte_vel=U, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=inlet,
+                     outlet_mode=outlet, parity_checks=0)
+    ref = oracle.Reference.from_geometry(g, tau=TAU, dt=DT, wall_mode=1, inlet_mode=inlet, outlet_mode=outlet)
+    prev = 0
+    for n in (1, 50, 300):
+        sol.step(n - prev)
+        ref.step(np.full(n - prev, U), nthreads=8)
+        prev = n
+        d = np.max(np.abs(sol.populations() - ref.f()))
+        print(f"step {n}: max |f - f_oracle| = {d:.3e}", flush=True)
+        assert d < 1e-12
+    w, acc, mag = sol.wss()
+    rw, racc, rmag = ref.wss()
+    assert np.max(np.abs(w - rw)) <= 1e-9 * np.max(np.abs(rw))
+    assert np.max(np.abs(mag - rmag)) <= 1e-9 * np.max(np.abs(rmag))
+
+
+def test_set_get_round_trip_and_init(c0_geometry):
+    rng = np.random.default_rng(5)
+    _, _, w = lbm.lattice(19)
+    f0 = w[None, :] * (1 + 0.05 * rng.uniform(-1, 1, (c0_geometry.n, 19)))
+    sol = lbm.Solver(c0_geometry, tau=TAU, dt=DT, cte_vel=U, wall_mode=lbm.WALL_HALFWAY_BB,
+                     inlet_mode=lbm.INLET_NEE_VELOCITY, outlet_mode=lbm.OUTLET_NEE_PRESSURE, parity_checks=0)
+    sol.set_populations(f0)
+    assert np.array_equal(sol.populations(), f0)
+    ref = oracle.Reference.from_geometry(c0_geometry, tau=TAU, dt=DT, wall_mode=1, inlet_mode=1, outlet_mode=1)
+    ref.set_f(f0)
+    sol.step(3)
+    ref.step(np.full(3, U))
+    assert np.max(np.abs(sol.populations() - ref.f())) < 1e-13
+
+
+def _poiseuille(prec, steps, R=8.0, L=12, g=2e-6, tau=0.8):
+    lab, origin, (ax, ay) = workloads.tube_mask(R, L)
+    geom = lbm.Geometry.from_labels(lab, origin, 1.0, periodic=(0, 0, 1), normals=False)
+    sol = lbm.Solver(geom, tau=tau, dt=DT, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NONE,
+                     outlet_mode=lbm.OUTLET_NONE, precision=prec, parity_checks=0, force=(0.0, 0.0, g),
+                     wss_every=0)
+    s0 = sol.reduce()
+    sol.step(steps)
+    return geom, sol, s0, (ax, ay), R, g, tau
+
+
+def test_periodic_poiseuille_closed_form():
+    geom, sol, s0, (ax, ay), R, g, tau = _poiseuille(lbm.PREC_F64, 6000)
+    ijk, _, _ = geom.nodes()
+    rho, u = sol.macroscopic()
+    # Guo forcing: the physical velocity is (j + F/2) / rho
+    uz = u[:, 2] + 0.5 * g / rho
+    r2 = (ijk[:, 0] - ax) ** 2 + (ijk[:, 1] - ay) ** 2
+    nu = (tau - 0.5) / 3.0
+    A = np.stack([np.ones_like(r2), r2], 1)
+    coef, *_ = np.linalg.lstsq(A, uz, rcond=None)
+    # u(r) = g / (4 nu) (R^2 - r^2): curvature fixed by g and nu
+    assert coef[1] == pytest.approx(-g / (4 * nu), rel=0.03)
+    fit = A @ coef
+    assert np.sqrt(np.mean((fit - uz) ** 2)) < 0.02 * uz.max()
+    # flow is axial and the density uniform
+    assert np.max(np.abs(u[:, :2])) < 1e-3 * uz.max()
+    s1 = sol.reduce()
+    assert abs(s1.mass - s0.mass) < 1e-10 * s0.mass
+
+
+def test_periodic_poiseuille_matches_oracle_early():
+    lab, origin, _ = workloads.tube_mask(5.0, 6)
+    geom = lbm.Geometry.from_labels(lab, origin, 1.0, periodic=(0, 0, 1), normals=False)
+    g = 1e-5
+    sol = lbm.Solver(geom, tau=0.9, dt=DT, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NONE,
+                     outlet_mode=lbm.OUTLET_NONE, parity_checks=0, force=(0.0, 0.0, g), wss_every=0)
+    ijk, kind, oid = geom.nodes()
+    ref = oracle.Reference(19, tuple(geom.info.grid), ijk, kind, oid, np.zeros((0, 3)), (0, 0, 0),
+                           np.zeros(0, np.int32), np.zeros((0, 3)), 0.9, 1.0, wall_mode=1, inlet_mode=2,
+                           outlet_mode=2, periodic=(0, 0, 1), force=(0, 0, g))
+    sol.step(200)
+    ref.step(np.zeros(200))
+    assert np.max(np.abs(sol.populations() - ref.f())) < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [(lbm.PREC_F32, 1e-4), (lbm.PREC_F16, 5e-2)])
+def test_reduced_precision_poiseuille(prec, tol):
+    _, s64, _, _, _, _, _ = _poiseuille(lbm.PREC_F64, 1500)
+    _, sp, _, _, _, _, _ = _poiseuille(prec, 1500)
+    _, u64 = s64.macroscopic()
+    _, up = sp.macroscopic()
+    err = np.linalg.norm(up - u64) / np.linalg.norm(u64)
+    print(f"precision {prec}: velocity rel L2 vs FP64 = {err:.3e}")
+    assert err < tol

@@ -32,6 +32,7 @@ import workloads  # noqa: E402
 METRIC = "MLUPS & %HBM roofline, FP32 vs FP64 at ≤1e-4 WSS err; adjoint/forward time"
 PEAK_HBM_GBS = 8000.0
 WO, PERIOD, U_MEAN, U_AMP = 4.0, 4000, 0.05, 0.025
+WSS_EVERY = 10  # near-wall stress sampled every 10th step (time average over 800 samples per 8000 steps)
 
 
 def womersley_basis(r):
@@ -80,7 +81,8 @@ def tau_for():
 
 def run_precision(geom, basis, prec, steps, warmup):
     s = lbm.Solver(geom, precision=prec, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NEE_VELOCITY,
-                   outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=tau_for(), dt=1e-5, parity_checks=0, wss_every=1)
+                   outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=tau_for(), dt=1e-5, parity_checks=0,
+                   wss_every=WSS_EVERY)
     s.set_inlet_profile(basis)
     if warmup:
         s.step_coefficients(coefficients(0, warmup))
@@ -145,6 +147,8 @@ def main():
         print(f"# {name}: {runs[name]['mlups']:.0f} MLUPS (wall {wall:.3f}s, events {ev_ms:.1f}ms)",
               file=sys.stderr, flush=True)
 
+    copy_gbps = lbm.copy_probe(1 << 30, iters=20)
+    print(f"# copy probe: {copy_gbps:.0f} GB/s (1 GiB read + write)", file=sys.stderr, flush=True)
     head = runs["f32"]
     sinfo = head["solver"].info()
     bytes_per_launch = N * sinfo.bytes_per_update
@@ -162,7 +166,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
                      "bytes_per_launch_algorithmic": bytes_per_launch,
-                     "launch_us": round(per_launch_s * 1e6, 3)},
+                     "launch_us": round(per_launch_s * 1e6, 3),
+                     "copy_probe_gbps": round(copy_gbps, 1)},
     }
     if "f64" in runs:
         out["fp64_mlups"] = round(runs["f64"]["mlups"], 1)

@@ -161,6 +161,10 @@ int lbm_geometry_get_nearwall(const lbm_geometry* g, int32_t* nodes, double* nor
 int lbm_geometry_get_bc_neighbour(const lbm_geometry* g, int32_t* bc_nbr);
 /* Outlet block normals (3 per outlet). */
 int lbm_geometry_get_outlet_normals(const lbm_geometry* g, double* normals);
+/* Triangles as read from the STL (Geometry::globalTriangles,
+ * GeometryReader.cpp:74-158): 12 doubles each, 3 vertices then the file
+ * normal. n_triangles in lbm_geometry_info. */
+int lbm_geometry_get_triangles(const lbm_geometry* g, double* tris);
 void lbm_geometry_destroy(lbm_geometry* g);
 
 /* ------------------------------------------------------------------------ */
@@ -270,6 +274,12 @@ typedef struct {
   double bytes_per_update;  /* algorithmic: 2 * Q * sizeof(population) */
 } lbm_solver_info;
 int lbm_get_solver_info(const lbm_solver* s, lbm_solver_info* out);
+
+/* Plain device copy of `bytes` (read once, written once) repeated `iters`
+ * times: the achievable HBM bandwidth reference (GB/s, read + write bytes)
+ * and a launch of known size for calibrating traffic counters
+ * (kernel copy_probe_kernel). No reference counterpart. */
+int lbm_copy_probe(int32_t device, int64_t bytes, int32_t iters, double* gbps);
 
 const char* lbm_last_error(void);
 

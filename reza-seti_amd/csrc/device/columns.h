@@ -82,7 +82,12 @@ struct ColWssArgs {
   const float* porosity;
 };
 
-hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st);
+// parts: COLS_MAIN (every slot but inlet/outlet nodes), COLS_BOUNDARY
+// (inlet/outlet nodes; they read only fs_in and write only their own slots,
+// so they may run concurrently with the main launch), or both in order.
+constexpr int COLS_MAIN = 1, COLS_BOUNDARY = 2, COLS_ALL = 3;
+hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st,
+                          int parts = COLS_ALL);
 hipError_t launch_columns_wss(int Q, int precision, bool force, const ColWssArgs& a, hipStream_t st);
 // Sum |tau|^2 and max |tau| (2 doubles into out2); partials needs 2 * ceil(n_nw / 256).
 hipError_t launch_columns_wss_stats(const ColWssArgs& a, double* partials, double* out2, hipStream_t st);

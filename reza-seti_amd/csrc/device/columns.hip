@@ -279,10 +279,11 @@ hipError_t launch_cols(const ColArgs& a, hipStream_t st) {
 }  // namespace
 
 template <class L, class S, int MODE, bool FORCE>
-static hipError_t launch_pair(const ColArgs& a, hipStream_t st) {
-  hipError_t e = launch_cols<L, S, MODE, FORCE>(a, st);
+static hipError_t launch_pair(const ColArgs& a, hipStream_t st, int parts) {
+  hipError_t e = hipSuccess;
+  if (parts & COLS_MAIN) e = launch_cols<L, S, MODE, FORCE>(a, st);
   if (e != hipSuccess) return e;
-  if (MODE != KMODE_INIT && a.n_bslots > 0) {
+  if ((parts & COLS_BOUNDARY) && MODE != KMODE_INIT && a.n_bslots > 0) {
     const unsigned nb = static_cast<unsigned>((a.n_bslots + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL((boundary_kernel<L, S, MODE, FORCE>), dim3(nb), dim3(BLOCK), 0, st, a);
     e = hipGetLastError();
@@ -291,31 +292,35 @@ static hipError_t launch_pair(const ColArgs& a, hipStream_t st) {
 }
 
 template <class L, class S>
-static hipError_t launch_cols_mode(int kmode, bool force, const ColArgs& a, hipStream_t st) {
+static hipError_t launch_cols_mode(int kmode, bool force, const ColArgs& a, hipStream_t st, int parts) {
   switch (kmode) {
     case KMODE_STEP:
-      return force ? launch_pair<L, S, KMODE_STEP, true>(a, st) : launch_pair<L, S, KMODE_STEP, false>(a, st);
+      return force ? launch_pair<L, S, KMODE_STEP, true>(a, st, parts)
+                   : launch_pair<L, S, KMODE_STEP, false>(a, st, parts);
     case KMODE_INIT:
-      return force ? launch_pair<L, S, KMODE_INIT, true>(a, st) : launch_pair<L, S, KMODE_INIT, false>(a, st);
+      return force ? launch_pair<L, S, KMODE_INIT, true>(a, st, parts)
+                   : launch_pair<L, S, KMODE_INIT, false>(a, st, parts);
     case KMODE_MATERIALIZE:
-      return launch_pair<L, S, KMODE_MATERIALIZE, false>(a, st);
+      return launch_pair<L, S, KMODE_MATERIALIZE, false>(a, st, parts);
   }
   return hipErrorInvalidValue;
 }
 
 template <class L>
-static hipError_t launch_cols_prec(int precision, int kmode, bool force, const ColArgs& a, hipStream_t st) {
+static hipError_t launch_cols_prec(int precision, int kmode, bool force, const ColArgs& a, hipStream_t st,
+                                   int parts) {
   switch (precision) {
-    case 0: return launch_cols_mode<L, double>(kmode, force, a, st);
-    case 1: return launch_cols_mode<L, float>(kmode, force, a, st);
-    case 2: return launch_cols_mode<L, __half>(kmode, force, a, st);
+    case 0: return launch_cols_mode<L, double>(kmode, force, a, st, parts);
+    case 1: return launch_cols_mode<L, float>(kmode, force, a, st, parts);
+    case 2: return launch_cols_mode<L, __half>(kmode, force, a, st, parts);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st) {
-  if (Q == 19) return launch_cols_prec<D3Q19>(precision, kmode, force, a, st);
-  if (Q == 27) return launch_cols_prec<D3Q27>(precision, kmode, force, a, st);
+hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st,
+                          int parts) {
+  if (Q == 19) return launch_cols_prec<D3Q19>(precision, kmode, force, a, st, parts);
+  if (Q == 27) return launch_cols_prec<D3Q27>(precision, kmode, force, a, st, parts);
   return hipErrorInvalidValue;
 }
 

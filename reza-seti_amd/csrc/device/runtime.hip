@@ -69,7 +69,9 @@ struct lbm_solver {
   int prec = 0;
   size_t esize = 8;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;        // concurrent boundary/WSS launches (default priority)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   int64_t N = 0, n_special = 0, n_special_pad = 0, n_bulk = 0, bulk_end = 0, npad = 0;
   std::vector<int32_t> slot_of;   // reference node -> slot
@@ -151,6 +153,9 @@ void free_all(lbm_solver* s) {
     if (p) (void)hipFree(p);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+  if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+  if (s->side) (void)hipStreamDestroy(s->side);
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -235,13 +240,24 @@ int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
     if (s->family == 1) {
       ColArgs c = col_args_for(s, s->cur);
       std::memcpy(c.coef, a.coef, sizeof(c.coef));
-      HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->stream));
-      if (a.wss_on) {
-        ColWssArgs w = s->wss_args;
-        w.fs_in = s->fs[s->cur];
-        w.porosity = s->porosity;
-        HIP_TRY(launch_columns_wss(s->Q, s->prec, force, w, s->stream));
+      // Inlet/outlet nodes and the near-wall stress only read this step's
+      // input and write their own outputs: they run on the side stream
+      // beside the main launch; the next step waits for them.
+      const bool side = c.n_bslots > 0 || a.wss_on;
+      if (side) {
+        HIP_TRY(hipEventRecord(s->ev_fork, s->stream));
+        HIP_TRY(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+        HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->side, COLS_BOUNDARY));
+        if (a.wss_on) {
+          ColWssArgs w = s->wss_args;
+          w.fs_in = s->fs[s->cur];
+          w.porosity = s->porosity;
+          HIP_TRY(launch_columns_wss(s->Q, s->prec, force, w, s->side));
+        }
+        HIP_TRY(hipEventRecord(s->ev_join, s->side));
       }
+      HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->stream, COLS_MAIN));
+      if (side) HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_join, 0));
     } else {
       HIP_TRY(launch_lbm(s->Q, s->prec, KMODE_STEP, force, a, s->stream));
     }
@@ -546,6 +562,10 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
 
   if (p->wall_mode == LBM_WALL_HALFWAY_BB) {
     s->family = 1;
+    if (hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(LBM_ERR_HIP, "side stream/event creation failed"));
     int rc = create_columns(s, t);
     if (rc == LBM_OK) rc = init_state(s, nullptr);
     if (rc == LBM_OK && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(LBM_ERR_HIP, "init failed");

@@ -268,6 +268,16 @@ int lbm_geometry_get_outlet_normals(const lbm_geometry* g, double* normals) {
   return LBM_OK;
 }
 
+int lbm_geometry_get_triangles(const lbm_geometry* g, double* tris) {
+  if (!g || !tris) return fail(LBM_ERR_ARG, "null argument");
+  for (size_t i = 0; i < g->tris.size(); ++i) {
+    const Triangle& T = g->tris[i];
+    const double v[12] = {T.x1, T.y1, T.z1, T.x2, T.y2, T.z2, T.x3, T.y3, T.z3, T.nx, T.ny, T.nz};
+    std::memcpy(tris + 12 * i, v, sizeof(v));
+  }
+  return LBM_OK;
+}
+
 void lbm_geometry_destroy(lbm_geometry* g) { delete g; }
 
 }  // extern "C"

@@ -98,6 +98,7 @@ SIGNATURES = [
     ("lbm_geometry_get_nearwall", [_P, _P, _P], C.c_int),
     ("lbm_geometry_get_bc_neighbour", [_P, _P], C.c_int),
     ("lbm_geometry_get_outlet_normals", [_P, _P], C.c_int),
+    ("lbm_geometry_get_triangles", [_P, _P], C.c_int),
     ("lbm_geometry_destroy", [_P], None),
     ("lbm_params_default", [_P], None),
     ("lbm_create", [_P, _P, _P], C.c_int),
@@ -117,8 +118,16 @@ SIGNATURES = [
     ("lbm_synchronize", [_P], C.c_int),
     ("lbm_destroy", [_P], None),
     ("lbm_get_solver_info", [_P, _P], C.c_int),
+    ("lbm_copy_probe", [C.c_int32, C.c_int64, C.c_int32, _P], C.c_int),
     ("lbm_last_error", [], C.c_char_p),
 ]
+
+
+def copy_probe(nbytes, iters=20, device=0):
+    """Achievable copy bandwidth in GB/s (read + write)."""
+    g = C.c_double()
+    _check(lib().lbm_copy_probe(device, int(nbytes), int(iters), C.byref(g)))
+    return g.value
 
 
 def lib():
@@ -256,6 +265,12 @@ class Geometry:
     def bc_neighbour(self):
         out = np.zeros(self.n, np.int32)
         _check(lib().lbm_geometry_get_bc_neighbour(self._h, _ptr(out)))
+        return out
+
+    def triangles(self):
+        out = np.zeros((self.info.n_triangles, 12), np.float64)
+        if self.info.n_triangles:
+            _check(lib().lbm_geometry_get_triangles(self._h, _ptr(out)))
         return out
 
     def outlet_normals(self):

@@ -73,12 +73,14 @@ def test_periodic_poiseuille_closed_form():
     uz = u[:, 2] + 0.5 * g / rho
     r2 = (ijk[:, 0] - ax) ** 2 + (ijk[:, 1] - ay) ** 2
     nu = (tau - 0.5) / 3.0
-    A = np.stack([np.ones_like(r2), r2], 1)
-    coef, *_ = np.linalg.lstsq(A, uz, rcond=None)
-    # u(r) = g / (4 nu) (R^2 - r^2): curvature fixed by g and nu
-    assert coef[1] == pytest.approx(-g / (4 * nu), rel=0.03)
+    # Fit away from the staircase wall (which bends the profile within ~2
+    # nodes of it): u(r) = g / (4 nu) (R^2 - r^2), curvature fixed by g, nu.
+    core = r2 < (R - 2.5) ** 2
+    A = np.stack([np.ones_like(r2), r2], 1)[core]
+    coef, *_ = np.linalg.lstsq(A, uz[core], rcond=None)
+    assert coef[1] == pytest.approx(-g / (4 * nu), rel=0.02)
     fit = A @ coef
-    assert np.sqrt(np.mean((fit - uz) ** 2)) < 0.02 * uz.max()
+    assert np.sqrt(np.mean((fit - uz[core]) ** 2)) < 0.01 * uz.max()
     # flow is axial and the density uniform
     assert np.max(np.abs(u[:, :2])) < 1e-3 * uz.max()
     s1 = sol.reduce()

@@ -1,4 +1,4 @@
-// Column-offset kernel family: argument blocks and launchers.
+// Column-offset kernel family: argument block and launchers.
 //
 // Used for half-way bounce-back walls (the reference has no node that keeps
 // stale populations there), with any inlet/outlet rule. Slots are the
@@ -8,7 +8,12 @@
 // neighbours (x - ox, y - oy, z); every pull source is one of them (or the
 // node itself) shifted by -c_z. Those 8 slots are stored as 16-bit offsets
 // from a per-64-slot base. Nodes that break the rule (missing links,
-// periodic wraps, inlet/outlet nodes) carry an explicit int32 row.
+// periodic wraps), inlet/outlet nodes and near-wall nodes carry an explicit
+// int32 row instead; a row also holds the node's boundary data.
+//
+// One launch per step: the first `b_blocks` blocks handle inlet/outlet nodes
+// (one thread each), the rest every other slot, XCD-remapped. Near-wall rows
+// accumulate the wall shear stress inside the same launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,6 +24,8 @@ namespace lbm {
 constexpr int NCOL = 8;              // lateral column offsets
 constexpr int CHUNK = 64;            // slots per base entry (one wavefront)
 constexpr uint16_t IRREGULAR = 0xFFFF;
+constexpr uint32_t ROW_KIND_MASK = 3u;  // NodeKind
+constexpr uint32_t ROW_NEARWALL = 4u;
 
 // Lateral offset index of a direction with (cx, cy), -1 for the own column.
 __host__ __device__ constexpr int col_of(int cx, int cy) {
@@ -38,7 +45,8 @@ struct ColArgs {
   void* fs_out;
   int64_t npad;          // SoA stride
   int32_t n;             // slots [0, n)
-  int32_t nblocks;
+  int32_t b_blocks;      // leading blocks for inlet/outlet slots
+  int32_t m_blocks;      // blocks for the other slots
   int32_t xcd_remap;
   int32_t xcd_first[8];
   int32_t xcd_offset[8];
@@ -48,10 +56,15 @@ struct ColArgs {
   int32_t nchunk;
   int32_t n_irr;
   const int32_t* irr_src;    // Q * n_irr pull sources (slot or -1 = bounce back)
-  const uint32_t* irr_kind;  // n_irr: NodeKind
+  const uint32_t* irr_kind;  // n_irr: NodeKind | ROW_NEARWALL
   const int32_t* irr_bcn;    // n_irr: slot behind an inlet/outlet, or -1
   const int32_t* irr_inlet;  // n_irr: inlet basis index, or -1
-  const int32_t* bslots;     // inlet/outlet slots handled by the boundary kernel
+  const double* irr_normal;  // 3 * n_irr (near-wall rows)
+  double* irr_rho_u;         // 4 * n_irr: rho (or rho - 1), u of the last collision (near-wall rows)
+  double* irr_wss;           // 5 * n_irr: |tau|, acc tau x/y/z, acc |tau| (near-wall rows)
+  double wss_C;
+  int32_t wss_on;
+  const int32_t* bslots;     // inlet/outlet slots
   int32_t n_bslots;
   int32_t n_inlet;
   int32_t n_modes;
@@ -68,28 +81,9 @@ struct ColArgs {
   double* f_out;          // materialize output
 };
 
-struct ColWssArgs {
-  const void* fs_in;      // input populations of the step just launched
-  int64_t npad;
-  int32_t n_nw;
-  const int32_t* slot;    // n_nw
-  const int32_t* src;     // Q * n_nw pull sources
-  const double* normal;   // 3 * n_nw
-  double* wss;            // 5 * n_nw: |tau|, acc tau x/y/z, acc |tau|
-  double wss_C;
-  double omega;
-  double force[3];
-  const float* porosity;
-};
-
-// parts: COLS_MAIN (every slot but inlet/outlet nodes), COLS_BOUNDARY
-// (inlet/outlet nodes; they read only fs_in and write only their own slots,
-// so they may run concurrently with the main launch), or both in order.
-constexpr int COLS_MAIN = 1, COLS_BOUNDARY = 2, COLS_ALL = 3;
-hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st,
-                          int parts = COLS_ALL);
-hipError_t launch_columns_wss(int Q, int precision, bool force, const ColWssArgs& a, hipStream_t st);
-// Sum |tau|^2 and max |tau| (2 doubles into out2); partials needs 2 * ceil(n_nw / 256).
-hipError_t launch_columns_wss_stats(const ColWssArgs& a, double* partials, double* out2, hipStream_t st);
+hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st);
+// Sum |tau|^2 and max |tau| over near-wall rows (2 doubles into out2);
+// partials needs 2 * ceil(n_irr / 256) doubles.
+hipError_t launch_columns_wss_stats(const ColArgs& a, double* partials, double* out2, hipStream_t st);
 
 }  // namespace lbm

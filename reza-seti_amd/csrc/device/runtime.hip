@@ -69,9 +69,7 @@ struct lbm_solver {
   int prec = 0;
   size_t esize = 8;
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;        // concurrent boundary/WSS launches (default priority)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   int64_t N = 0, n_special = 0, n_special_pad = 0, n_bulk = 0, bulk_end = 0, npad = 0;
   std::vector<int32_t> slot_of;   // reference node -> slot
@@ -116,13 +114,12 @@ struct lbm_solver {
   int32_t* irr_bcn = nullptr;
   int32_t* irr_inlet = nullptr;
   int32_t* bslots = nullptr;
-  int32_t* nw_slot = nullptr;
-  int32_t* nw_src = nullptr;
-  double* nw_normal = nullptr;
-  double* nw_wss = nullptr;
+  double* irr_normal = nullptr;
+  double* irr_rho_u = nullptr;
+  double* irr_wss = nullptr;
   int64_t n_irr = 0;
+  std::vector<int32_t> nw_rows;      // near-wall list order -> irregular row
   ColArgs cbase_args{};
-  ColWssArgs wss_args{};
 };
 
 namespace {
@@ -148,14 +145,11 @@ void free_all(lbm_solver* s) {
   void* ptrs[] = {s->fs[0], s->fs[1], s->fprev[0], s->fprev[1], s->src, s->sflags, s->bc_nbr, s->inlet_slot,
                   s->inlet_basis, s->normal, s->rho_u, s->wss, s->porosity, s->err, s->f_tmp, s->partials,
                   s->red_out, s->idx16, s->cbase, s->irr_src, s->irr_kind, s->irr_bcn, s->irr_inlet,
-                  s->bslots, s->nw_slot, s->nw_src, s->nw_normal, s->nw_wss};
+                  s->bslots, s->irr_normal, s->irr_rho_u, s->irr_wss};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
-  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
-  if (s->ev_join) (void)hipEventDestroy(s->ev_join);
-  if (s->side) (void)hipStreamDestroy(s->side);
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -240,24 +234,8 @@ int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
     if (s->family == 1) {
       ColArgs c = col_args_for(s, s->cur);
       std::memcpy(c.coef, a.coef, sizeof(c.coef));
-      // Inlet/outlet nodes and the near-wall stress only read this step's
-      // input and write their own outputs: they run on the side stream
-      // beside the main launch; the next step waits for them.
-      const bool side = c.n_bslots > 0 || a.wss_on;
-      if (side) {
-        HIP_TRY(hipEventRecord(s->ev_fork, s->stream));
-        HIP_TRY(hipStreamWaitEvent(s->side, s->ev_fork, 0));
-        HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->side, COLS_BOUNDARY));
-        if (a.wss_on) {
-          ColWssArgs w = s->wss_args;
-          w.fs_in = s->fs[s->cur];
-          w.porosity = s->porosity;
-          HIP_TRY(launch_columns_wss(s->Q, s->prec, force, w, s->side));
-        }
-        HIP_TRY(hipEventRecord(s->ev_join, s->side));
-      }
-      HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->stream, COLS_MAIN));
-      if (side) HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_join, 0));
+      c.wss_on = a.wss_on;
+      HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->stream));
     } else {
       HIP_TRY(launch_lbm(s->Q, s->prec, KMODE_STEP, force, a, s->stream));
     }
@@ -328,11 +306,14 @@ int create_columns(lbm_solver* s, const Tables& t) {
     if (qcol[o] < 0) return fail(LBM_ERR_ARG, "lattice lacks an in-plane direction");
   }
   const bool inl_b = p.inlet_mode != LBM_INLET_NONE, out_b = p.outlet_mode != LBM_OUTLET_NONE;
+  // Near-wall nodes keep their stress data in a row, so they are irregular.
+  std::vector<int32_t> nw_index(N, -1);
+  for (size_t i = 0; i < t.nearwall.size(); ++i) nw_index[t.nearwall[i]] = static_cast<int32_t>(i);
   std::vector<uint8_t> reg(N, 0);
   std::vector<int32_t> col(static_cast<size_t>(NCOL) * N, -1);
   for (int64_t n = 0; n < N; ++n) {
     for (int o = 0; o < NCOL; ++o) col[static_cast<size_t>(o) * N + n] = srcq(qcol[o], n);
-    bool r = !((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b));
+    bool r = !((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b)) && nw_index[n] < 0;
     for (int q = 0; q < Q && r; ++q) {
       const int32_t sq = srcq(q, n);
       const int o = col_of(L.cx[q], L.cy[q]);
@@ -377,13 +358,21 @@ int create_columns(lbm_solver* s, const Tables& t) {
   std::vector<int32_t> isrc(static_cast<size_t>(Q) * std::max<int64_t>(nirr, 1), -1);
   std::vector<uint32_t> ikind(std::max<int64_t>(nirr, 1), 0);
   std::vector<int32_t> ibcn(std::max<int64_t>(nirr, 1), -1), iinl(std::max<int64_t>(nirr, 1), -1);
+  std::vector<double> inrm(3 * static_cast<size_t>(std::max<int64_t>(nirr, 1)), 0.0);
   std::vector<int32_t> bsl;
+  s->nw_rows.assign(t.nearwall.size(), -1);
   for (int64_t r = 0; r < nirr; ++r) {
     const int32_t n = rows[r];
     for (int q = 0; q < Q; ++q) isrc[static_cast<size_t>(q) * nirr + r] = srcq(q, n);
     ikind[r] = t.kind[n];
     ibcn[r] = t.bc_nbr[n];
     iinl[r] = inlet_index[n];
+    if (nw_index[n] >= 0) {
+      const int32_t i = nw_index[n];
+      ikind[r] |= ROW_NEARWALL;
+      s->nw_rows[i] = static_cast<int32_t>(r);
+      for (int d = 0; d < 3; ++d) inrm[static_cast<size_t>(d) * nirr + r] = t.normals[3 * static_cast<size_t>(i) + d];
+    }
     idx[n] = IRREGULAR;
     idx[static_cast<size_t>(npad) + n] = static_cast<uint16_t>(r & 0xFFFF);
     idx[static_cast<size_t>(2) * npad + n] = static_cast<uint16_t>(uint32_t(r) >> 16);
@@ -404,25 +393,23 @@ int create_columns(lbm_solver* s, const Tables& t) {
     }
     if (idx[n] == IRREGULAR) return fail(LBM_ERR_STATE, "regular slot carries the irregular marker");
   }
-  const int64_t nnw = static_cast<int64_t>(t.nearwall.size());
-  std::vector<int32_t> nwsrc(static_cast<size_t>(Q) * std::max<int64_t>(nnw, 1), -1);
-  for (int64_t i = 0; i < nnw; ++i)
-    for (int q = 0; q < Q; ++q) nwsrc[static_cast<size_t>(q) * nnw + i] = srcq(q, t.nearwall[i]);
+  for (int32_t r : s->nw_rows)
+    if (r < 0) return fail(LBM_ERR_STATE, "near-wall node without a row");
 
   int rc = LBM_OK;
   const size_t pop_bytes = static_cast<size_t>(Q) * npad * s->esize;
   const size_t ninl = std::max<size_t>(t.inlet_nodes.size(), 1);
+  const size_t nrow = static_cast<size_t>(std::max<int64_t>(nirr, 1));
   const int64_t nblk = npad / BLOCK;
   if ((rc = dalloc_raw(s, &s->fs[0], pop_bytes)) || (rc = dalloc_raw(s, &s->fs[1], pop_bytes)) ||
       (rc = dalloc(s, &s->idx16, idx.size())) || (rc = dalloc(s, &s->cbase, cb.size())) ||
       (rc = dalloc(s, &s->irr_src, isrc.size())) || (rc = dalloc(s, &s->irr_kind, ikind.size())) ||
       (rc = dalloc(s, &s->irr_bcn, ibcn.size())) || (rc = dalloc(s, &s->irr_inlet, iinl.size())) ||
       (rc = dalloc(s, &s->bslots, std::max<size_t>(bsl.size(), 1))) ||
-      (rc = dalloc(s, &s->nw_slot, std::max<int64_t>(nnw, 1))) || (rc = dalloc(s, &s->nw_src, nwsrc.size())) ||
-      (rc = dalloc(s, &s->nw_normal, 3 * std::max<int64_t>(nnw, 1))) ||
-      (rc = dalloc(s, &s->nw_wss, 5 * std::max<int64_t>(nnw, 1))) || (rc = dalloc(s, &s->err, 1)) ||
+      (rc = dalloc(s, &s->irr_normal, 3 * nrow)) || (rc = dalloc(s, &s->irr_rho_u, 4 * nrow)) ||
+      (rc = dalloc(s, &s->irr_wss, 5 * nrow)) || (rc = dalloc(s, &s->err, 1)) ||
       (rc = dalloc(s, &s->inlet_basis, MAX_MODES * ninl)) ||
-      (rc = dalloc(s, &s->partials, static_cast<size_t>(4 * nblk + 2 * ((nnw + BLOCK - 1) / BLOCK) + 8))) ||
+      (rc = dalloc(s, &s->partials, static_cast<size_t>(4 * nblk + 2 * ((nrow + BLOCK - 1) / BLOCK) + 8))) ||
       (rc = dalloc(s, &s->red_out, 8)))
     return rc;
   std::vector<double> basis(MAX_MODES * ninl, 0.0);
@@ -433,10 +420,10 @@ int create_columns(lbm_solver* s, const Tables& t) {
   if (up(s->idx16, idx.data(), idx.size() * 2) || up(s->cbase, cb.data(), cb.size() * 4) ||
       up(s->irr_src, isrc.data(), isrc.size() * 4) || up(s->irr_kind, ikind.data(), ikind.size() * 4) ||
       up(s->irr_bcn, ibcn.data(), ibcn.size() * 4) || up(s->irr_inlet, iinl.data(), iinl.size() * 4) ||
-      up(s->bslots, bsl.data(), bsl.size() * 4) || up(s->nw_slot, t.nearwall.data(), nnw * 4) ||
-      up(s->nw_src, nwsrc.data(), nwsrc.size() * 4) || up(s->nw_normal, t.normals.data(), 3 * nnw * 8) ||
+      up(s->bslots, bsl.data(), bsl.size() * 4) || up(s->irr_normal, inrm.data(), inrm.size() * 8) ||
       up(s->inlet_basis, basis.data(), basis.size() * 8) ||
-      hipMemsetAsync(s->nw_wss, 0, 5 * std::max<int64_t>(nnw, 1) * 8, s->stream) ||
+      hipMemsetAsync(s->irr_wss, 0, 5 * nrow * 8, s->stream) ||
+      hipMemsetAsync(s->irr_rho_u, 0, 4 * nrow * 8, s->stream) ||
       hipMemsetAsync(s->err, 0, 4, s->stream) || hipMemsetAsync(s->fs[0], 0, pop_bytes, s->stream) ||
       hipMemsetAsync(s->fs[1], 0, pop_bytes, s->stream))
     return fail(LBM_ERR_HIP, "upload failed");
@@ -450,9 +437,10 @@ int create_columns(lbm_solver* s, const Tables& t) {
   std::memset(&c, 0, sizeof(c));
   c.npad = npad;
   c.n = static_cast<int32_t>(N);
-  c.nblocks = static_cast<int32_t>(nblk);
+  c.b_blocks = static_cast<int32_t>((bsl.size() + BLOCK - 1) / BLOCK);
+  c.m_blocks = static_cast<int32_t>(nblk);
   c.xcd_remap = 1;
-  xcd_tables(0, c.nblocks, c.xcd_first, c.xcd_offset);
+  xcd_tables(c.b_blocks, c.m_blocks, c.xcd_first, c.xcd_offset);
   c.idx16 = s->idx16;
   c.cbase = s->cbase;
   c.nchunk = static_cast<int32_t>(nchunk);
@@ -461,6 +449,10 @@ int create_columns(lbm_solver* s, const Tables& t) {
   c.irr_kind = s->irr_kind;
   c.irr_bcn = s->irr_bcn;
   c.irr_inlet = s->irr_inlet;
+  c.irr_normal = s->irr_normal;
+  c.irr_rho_u = s->irr_rho_u;
+  c.irr_wss = s->irr_wss;
+  c.wss_C = wss_C;
   c.bslots = s->bslots;
   c.n_bslots = static_cast<int32_t>(bsl.size());
   c.n_inlet = static_cast<int32_t>(ninl);
@@ -472,18 +464,6 @@ int create_columns(lbm_solver* s, const Tables& t) {
   c.outlet_rho = p.outlet_rho;
   c.omega = omega;
   for (int i = 0; i < 3; ++i) c.force[i] = p.force[i];
-
-  ColWssArgs& w = s->wss_args;
-  std::memset(&w, 0, sizeof(w));
-  w.npad = npad;
-  w.n_nw = static_cast<int32_t>(nnw);
-  w.slot = s->nw_slot;
-  w.src = s->nw_src;
-  w.normal = s->nw_normal;
-  w.wss = s->nw_wss;
-  w.wss_C = wss_C;
-  w.omega = omega;
-  for (int i = 0; i < 3; ++i) w.force[i] = p.force[i];
 
   // Generic fields used by the shared moments kernel and init_state.
   StepArgs& a = s->base;
@@ -562,10 +542,6 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
 
   if (p->wall_mode == LBM_WALL_HALFWAY_BB) {
     s->family = 1;
-    if (hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess)
-      return bail(fail(LBM_ERR_HIP, "side stream/event creation failed"));
     int rc = create_columns(s, t);
     if (rc == LBM_OK) rc = init_state(s, nullptr);
     if (rc == LBM_OK && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(LBM_ERR_HIP, "init failed");
@@ -863,17 +839,18 @@ int lbm_get_wss(lbm_solver* s, double* wss, double* acc_tau, double* acc_mag) {
   if (!s) return fail(LBM_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->prm.device));
   if (s->family == 1) {
-    const int64_t m = static_cast<int64_t>(s->nw_slots.size());
+    const int64_t m = s->n_irr;
     std::vector<double> h(static_cast<size_t>(5 * m));
     if (m) {
-      HIP_TRY(hipMemcpyAsync(h.data(), s->nw_wss, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+      HIP_TRY(hipMemcpyAsync(h.data(), s->irr_wss, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
       HIP_TRY(hipStreamSynchronize(s->stream));
     }
-    for (int64_t i = 0; i < m; ++i) {
-      if (wss) wss[i] = h[i];
+    for (size_t i = 0; i < s->nw_rows.size(); ++i) {
+      const int64_t r = s->nw_rows[i];
+      if (wss) wss[i] = h[r];
       if (acc_tau)
-        for (int a = 0; a < 3; ++a) acc_tau[3 * i + a] = h[(1 + a) * m + i];
-      if (acc_mag) acc_mag[i] = h[4 * m + i];
+        for (int a = 0; a < 3; ++a) acc_tau[3 * i + a] = h[(1 + a) * m + r];
+      if (acc_mag) acc_mag[i] = h[4 * m + r];
     }
     return LBM_OK;
   }
@@ -944,7 +921,7 @@ int lbm_reduce(lbm_solver* s, lbm_stats* out) {
   StepArgs a = args_for(s, s->cur);
   HIP_TRY(launch_moments(s->Q, s->prec, a, s->partials, s->red_out, s->stream));
   if (s->family == 1)
-    HIP_TRY(launch_columns_wss_stats(s->wss_args, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
+    HIP_TRY(launch_columns_wss_stats(s->cbase_args, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
   else
     HIP_TRY(launch_wss_stats(a, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
   double h[6];

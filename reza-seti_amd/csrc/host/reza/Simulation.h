@@ -1,0 +1,235 @@
+// C++ drop-in for the reference's class surface over the C ABI.
+//
+// Mirrors the construction chain of LBM.cpp:95-111
+//   parseInputFile -> LatticeModel -> Geometry -> SubDomain -> Simulation
+// and the public methods of Simulation (Simulation.h:74-87), VelocitySet /
+// LatticeModel (VelocitySet.h:60-65, LatticeModel.h:50-54). Everything that
+// computes runs behind include/lbm_c_api.h on the GPU; these classes only hold
+// handles and forward calls.
+//
+// One reference iteration is one fused launch. Collision() issues it (the
+// launch streams, applies the inlet/outlet/wall rules, accumulates the wall
+// shear stress of the current iteration and performs the collision of the
+// next one). Communicate, Streaming, ApplyZouHeBc, ApplyOutletBc, ApplyWallBc
+// and ComputeWallShearStress are therefore folded into it and do nothing on
+// their own; calling the reference sequence in order gives the reference
+// iteration.
+#pragma once
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../../include/lbm_c_api.h"
+#include "../config.h"
+
+namespace reza {
+
+using SimulationParameters = lbm::SimulationParameters;
+
+inline void parseInputFile(const std::string& fileName, SimulationParameters& p) {
+  lbm::parseInputFile(fileName, p);
+}
+
+inline void check(int rc, const char* what) {
+  if (rc != LBM_OK) throw std::runtime_error(std::string(what) + ": " + lbm_last_error());
+}
+
+// VelocitySet / LatticeModel getters (VelocitySet.cpp:52-82, LatticeModel.cpp:149-191).
+class LatticeModel {
+ public:
+  explicit LatticeModel(int q = 19) : q_(q), c_(3 * q), opp_(q), w_(q) {
+    check(lbm_lattice(q, c_.data(), opp_.data(), w_.data()), "lbm_lattice");
+  }
+  double GetWeight(int dir) const { return w_[dir]; }
+  std::vector<int> GetDirection(int dir) const { return {c_[3 * dir], c_[3 * dir + 1], c_[3 * dir + 2]}; }
+  double Get_c_square(int) const { return 1.0 / 3.0; }
+  int Get_nDimensions() const { return 3; }
+  int Get_nDirections() const { return q_; }
+  int OppositeDirection(int dir) const { return opp_[dir]; }
+
+ private:
+  int q_;
+  std::vector<int32_t> c_, opp_;
+  std::vector<double> w_;
+};
+
+// Geometry: STL -> grid -> labels -> boundary sets (ComplexGeometry.cpp:48-73).
+class Geometry {
+ public:
+  Geometry(std::shared_ptr<LatticeModel> velSet, SimulationParameters& simParams, const std::string& config_path)
+      : velSet_(std::move(velSet)), simParams(simParams) {
+    lbm_geometry_options o;
+    lbm_geometry_options_default(&o);
+    o.q = velSet_->Get_nDirections();
+    lbm_geometry* g = nullptr;
+    check(lbm_geometry_from_config(config_path.c_str(), &o, &g), "Geometry");
+    handle_.reset(g, lbm_geometry_destroy);
+    check(lbm_geometry_get_info(g, &info_), "Geometry info");
+    spacing = info_.spacing;
+    GeomSize = {info_.grid[0], info_.grid[1], info_.grid[2]};
+  }
+  lbm_geometry* handle() const { return handle_.get(); }
+  const lbm_geometry_info& info() const { return info_; }
+
+  std::shared_ptr<LatticeModel> velSet_;
+  SimulationParameters simParams;
+  double spacing = 0.0;
+  std::vector<int> GeomSize;
+
+ private:
+  std::shared_ptr<lbm_geometry> handle_;
+  lbm_geometry_info info_{};
+};
+
+// SubDomain: node lists of one rank (Subdomain.cpp:43-62). On one GPU the
+// whole lattice is one subdomain.
+class SubDomain {
+ public:
+  explicit SubDomain(Geometry& g) : Geometry_(g) {}
+  int64_t NumberOfNodes() const { return Geometry_.info().n_nodes; }
+  Geometry& Geometry_;
+};
+
+class Simulation {
+ public:
+  // Simulation.cpp:44-58: set up and run the whole loop.
+  Simulation(SubDomain& sub, SimulationParameters& simParams) : Simulation(sub, simParams, nullptr) { Iterate(); }
+
+  // Set-up only, with solver parameters beyond the reference's (precision,
+  // boundary modes); `extra` may be null.
+  Simulation(SubDomain& sub, SimulationParameters& simParams, const lbm_params* extra)
+      : SubDomain_(sub), simParams(simParams) {
+    lbm_params p;
+    if (extra) p = *extra;
+    else lbm_params_default(&p);
+    p.tau = simParams.tau;
+    p.dt = simParams.dt;
+    p.cte_vel = simParams.CteVel;
+    p.flow_type = simParams.FlowType == "Pulsatile" ? LBM_FLOW_PULSATILE : LBM_FLOW_CONSTANT;
+    SetRelaxation(simParams.tau);
+    m_dt = simParams.dt;
+    lbm_solver* s = nullptr;
+    check(lbm_create(sub.Geometry_.handle(), &p, &s), "Simulation");
+    solver_.reset(s, lbm_destroy);
+  }
+
+  // One reference iteration per call (see the file comment).
+  void Collision() { check(lbm_step(solver_.get(), 1), "Collision"); }
+  void Communicate() {}      // one GPU: nothing to exchange (Simulation.cpp:270-516)
+  void Streaming() {}        // folded into Collision()
+  void ApplyZouHeBc() {}     // folded into Collision()
+  void ApplyOutletBc() {}    // folded into Collision()
+  void ApplyWallBc() {}      // folded into Collision()
+  void ComputeWallShearStress(int) {}  // accumulated by the fused launch
+
+  // Simulation::ComputeOSIAndTAWSS (Simulation.cpp:804-826).
+  void ComputeOSIAndTAWSS(int) {
+    const int64_t n = SubDomain_.Geometry_.info().n_nearwall;
+    osi.resize(n);
+    tawss.resize(n);
+    check(lbm_get_osi_tawss(solver_.get(), osi.data(), tawss.data()), "ComputeOSIAndTAWSS");
+  }
+
+  // Simulation::getInletVelocity (Simulation.cpp:519-534, pulsatile.h:55-69).
+  double getInletVelocity(double x, double period) {
+    double vel = 0.0;
+    const double t = std::fmod(x, period);
+    if (t >= 0 && t < (0.1 * period))
+      vel = (-8.579e+08 * std::pow(t, 8) + 4.059e+08 * std::pow(t, 7) - 7.817e+07 * std::pow(t, 6) +
+             7.994e+06 * std::pow(t, 5) - 4.835e+05 * std::pow(t, 4) + 1.782e+04 * std::pow(t, 3) -
+             321.6 * std::pow(t, 2) + 3.215 * t + 0.1423);
+    else if (t >= (0.1 * period) && t < (0.3 * period))
+      vel = (-4.32e+06 * std::pow(t, 8) + 7.94e+06 * std::pow(t, 7) - 6.298e+06 * std::pow(t, 6) +
+             2.82e+06 * std::pow(t, 5) - 7.809e+05 * std::pow(t, 4) + 1.37e+05 * std::pow(t, 3) -
+             1.484e+04 * std::pow(t, 2) + 901.9 * t - 22.87);
+    else if (t >= (0.3 * period) && t < (0.9 * period))
+      vel = (-5.123 * std::pow(t, 4) + 14.15 * std::pow(t, 3) - 14.21 * std::pow(t, 2) + 5.865 * t - 0.5696);
+    return vel;
+  }
+
+  void SetRelaxation(double tau) { m_relaxation = tau; }
+
+  // Simulation::Iterate (Simulation.cpp:538-618): loop condition, output
+  // cadence and progress line of the reference.
+  void Iterate() {
+    double total_runtime = 0.0;
+    int iter = 0;
+    int systole_count = 0, diastole_count = 0;
+    const double tolerance = m_dt / 2;
+    const double period = 0.9;
+    while ((simParams.FlowType == "Constant" && iter < simParams.MaxIter) ||
+           (simParams.FlowType == "Pulsatile" && total_runtime <= 2.7)) {
+      Collision();
+      Communicate();
+      Streaming();
+      ApplyZouHeBc();
+      ApplyOutletBc();
+      ApplyWallBc();
+      ComputeWallShearStress(iter);
+      if (simParams.FlowType == "Pulsatile") {
+        ComputeOSIAndTAWSS(iter);
+        const double time_in_cycle = std::fmod(total_runtime, period);
+        if (std::fabs(time_in_cycle - simParams.Systole) < tolerance)
+          WriteOutput(iter, "Systole_" + std::to_string(++systole_count));
+        else if (std::fabs(time_in_cycle - simParams.Diastole) < tolerance)
+          WriteOutput(iter, "Diastole_" + std::to_string(++diastole_count));
+      }
+      if (simParams.FlowType == "Constant" && iter > 0 && simParams.IterToWrite > 0 &&
+          iter % simParams.IterToWrite == 0)
+        WriteOutput(iter, "Iteration_" + std::to_string(iter));
+      if (simParams.FlowType == "Pulsatile") total_runtime += m_dt;
+      if (verbose) std::printf("Iteration %d has been performed successfully\n", iter);
+      iter++;
+    }
+    iterations_done = iter;
+  }
+
+  // Near-wall attributes in the column layout of
+  // OutputFile::WriteSurfaceAttribute (OutputFile.cpp:296-309): X,Y,Z,WSS
+  // [,OSI,TAWSS],VelMag; written to <phase>_<iter>_FlowHemodynamics.csv.
+  void WriteOutput(int iter, const std::string& phase_name) {
+    if (output_dir.empty()) return;
+    const lbm_geometry_info& gi = SubDomain_.Geometry_.info();
+    const int64_t n = gi.n_nearwall, N = gi.n_nodes;
+    std::vector<int32_t> nodes(n), ijk(3 * N);
+    std::vector<double> nrm(3 * n), wss(n), rho(N), u(3 * N);
+    check(lbm_geometry_get_nearwall(SubDomain_.Geometry_.handle(), nodes.data(), nrm.data()), "nearwall");
+    check(lbm_geometry_get_nodes(SubDomain_.Geometry_.handle(), ijk.data(), nullptr, nullptr), "nodes");
+    check(lbm_get_wss(solver_.get(), wss.data(), nullptr, nullptr), "wss");
+    check(lbm_get_macroscopic(solver_.get(), rho.data(), u.data()), "macroscopic");
+    const bool pulsatile = simParams.FlowType == "Pulsatile";
+    if (pulsatile) ComputeOSIAndTAWSS(iter);
+    std::ofstream f(output_dir + "/" + phase_name + "_" + std::to_string(iter) + "_FlowHemodynamics.csv");
+    f << (pulsatile ? "X,Y,Z,WSS,OSI,TAWSS,VelMag\n" : "X,Y,Z,WSS,VelMag\n");
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t m = nodes[i];
+      const double x = gi.bbox[0] + ijk[3 * m] * gi.spacing, y = gi.bbox[2] + ijk[3 * m + 1] * gi.spacing,
+                   z = gi.bbox[4] + ijk[3 * m + 2] * gi.spacing;
+      const double vm = std::sqrt(u[3 * m] * u[3 * m] + u[3 * m + 1] * u[3 * m + 1] + u[3 * m + 2] * u[3 * m + 2]);
+      f << x << "," << y << "," << z << "," << wss[i];
+      if (pulsatile) f << "," << osi[i] << "," << tawss[i];
+      f << "," << vm << "\n";
+    }
+  }
+
+  lbm_solver* solver() const { return solver_.get(); }
+
+  SubDomain& SubDomain_;
+  SimulationParameters simParams;
+  double m_relaxation = 1.0;
+  double m_dt = 0.0;
+  std::vector<double> osi, tawss;
+  std::string output_dir;  // empty: WriteOutput writes nothing
+  bool verbose = false;
+  int iterations_done = 0;
+
+ private:
+  std::shared_ptr<lbm_solver> solver_;
+};
+
+}  // namespace reza

@@ -1,0 +1,45 @@
+"""lbm_run: the reference driver (LBM.cpp main) over the C ABI, through the
+C++ drop-in classes (Simulation / SubDomain / Geometry / LatticeModel)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import workloads
+from conftest import ROOT
+
+EXE = os.path.join(ROOT, "reza-seti_amd", "lbm_run")
+
+
+def test_driver_reports_a_missing_config():
+    if not os.path.exists(EXE):
+        pytest.skip("lbm_run not built")
+    r = subprocess.run([EXE, "/nonexistent/config.txt"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Unable to open input file" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_runs_the_reference_loop_bit_exact(tmp_path):
+    import lbm
+
+    cfg = workloads.c0_tube(str(tmp_path), radius=6.0, length=30.0, cte_vel=0.03, max_iter=150)
+    dump = tmp_path / "f.bin"
+    out = tmp_path / "out"
+    out.mkdir()
+    text = open(cfg).read().replace("IterToWrite: 100000", "IterToWrite: 100")
+    open(cfg, "w").write(text)
+    r = subprocess.run([EXE, cfg, "--dump", str(dump), "--out", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "Iterations: 150" in r.stdout
+    geom = lbm.Geometry.from_config(cfg)
+    f = np.fromfile(dump, dtype=np.float64).reshape(geom.n, 19)
+    ref = oracle.Reference.from_geometry(geom, tau=0.8, dt=1e-5)
+    ref.step(np.full(150, 0.03))
+    assert np.array_equal(f, ref.f())
+    csv = out / "Iteration_100_100_FlowHemodynamics.csv"
+    lines = csv.read_text().splitlines()
+    assert lines[0] == "X,Y,Z,WSS,VelMag" and len(lines) == geom.info.n_nearwall + 1

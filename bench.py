@@ -32,7 +32,7 @@ import workloads  # noqa: E402
 METRIC = "MLUPS & %HBM roofline, FP32 vs FP64 at ≤1e-4 WSS err; adjoint/forward time"
 PEAK_HBM_GBS = 8000.0
 WO, PERIOD, U_MEAN, U_AMP = 4.0, 4000, 0.05, 0.025
-WSS_EVERY = 10  # near-wall stress sampled every 10th step (time average over 800 samples per 8000 steps)
+WSS_EVERY = 1  # wall shear stress and its time sums every step, as Simulation::Iterate does
 
 
 def womersley_basis(r):
@@ -118,6 +118,23 @@ def cpu_baseline(geom, budget_s=12.0):
                       f"reference step (bounce-back walls, NEE inlet/outlet), OpenMP on {ncores} host threads"}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "columns_step_c1_bifurcation_f32_traffic.md")
+
+
+def profiled_traffic():
+    """HBM bytes per FP32 step launch measured by the two rocprofv3 counter
+    passes (FETCH_SIZE, WRITE_SIZE, copy-kernel calibrated) recorded in the
+    profile summary; None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def rel_l2(a, b):
     d = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / d) if d > 0 else float("nan")
@@ -164,7 +181,7 @@ def main():
                    "inlet": f"Womersley Wo={WO} period {PERIOD} steps, U_mean {U_MEAN}",
                    "outlet": "NEE pressure rho=1", "walls": "half-way bounce-back", "tau": round(tau_for(), 6)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": profiled_traffic(),
                      "bytes_per_launch_algorithmic": bytes_per_launch,
                      "launch_us": round(per_launch_s * 1e6, 3),
                      "copy_probe_gbps": round(copy_gbps, 1)},

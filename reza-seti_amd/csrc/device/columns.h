@@ -1,19 +1,24 @@
-// Column-offset kernel family: argument block and launchers.
+// Column-segment kernel family: argument block and launchers.
 //
 // Used for half-way bounce-back walls (the reference has no node that keeps
 // stale populations there), with any inlet/outlet rule. Slots are the
-// reference node order itself (i outer, k inner), so the node at
-// (x', y', z - 1) sits one slot before (x', y', z) whenever both exist. A
-// regular node therefore needs only the slots of its 8 lateral column
-// neighbours (x - ox, y - oy, z); every pull source is one of them (or the
-// node itself) shifted by -c_z. Those 8 slots are stored as 16-bit offsets
-// from a per-64-slot base. Nodes that break the rule (missing links,
-// periodic wraps), inlet/outlet nodes and near-wall nodes carry an explicit
-// int32 row instead; a row also holds the node's boundary data.
+// reference node order itself (i outer, k inner), so a run of nodes with the
+// same (i, j) and consecutive k -- a column segment -- occupies consecutive
+// slots, and the node at (x, y, z - 1) sits one slot before (x, y, z). For
+// every segment and each of the 8 lateral offsets (ox, oy), the slot of the
+// node at (x - ox, y - oy, z) is the node's own slot plus one per-segment
+// delta. A pull source is then p + delta(o) - c_z (or p - c_z in the own
+// column), and a node needs no per-node index data at all.
+//
+// Per 64-slot chunk (one wavefront) a ChunkMeta holds bit masks of the slots
+// that start a segment, that miss a link (bounce-back mask in `bbm`), that
+// carry an explicit pull row (periodic wraps, segment conflicts, inlet/outlet
+// nodes) and that are near-wall (stress data); a lane finds its entry in each
+// compact array by a population count over the lower lanes.
 //
 // One launch per step: the first `b_blocks` blocks handle inlet/outlet nodes
-// (one thread each), the rest every other slot, XCD-remapped. Near-wall rows
-// accumulate the wall shear stress inside the same launch.
+// (one thread each), the rest every other slot, XCD-remapped. Near-wall
+// nodes accumulate the wall shear stress inside the same launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,10 +27,8 @@
 namespace lbm {
 
 constexpr int NCOL = 8;              // lateral column offsets
-constexpr int CHUNK = 64;            // slots per base entry (one wavefront)
-constexpr uint16_t IRREGULAR = 0xFFFF;
-constexpr uint32_t ROW_KIND_MASK = 3u;  // NodeKind
-constexpr uint32_t ROW_NEARWALL = 4u;
+constexpr int CHUNK = 64;            // slots per ChunkMeta (one wavefront)
+constexpr uint32_t ROW_KIND_MASK = 3u;  // NodeKind of an explicit row
 
 // Lateral offset index of a direction with (cx, cy), -1 for the own column.
 __host__ __device__ constexpr int col_of(int cx, int cy) {
@@ -40,6 +43,31 @@ __host__ __device__ constexpr int col_of(int cx, int cy) {
                                   : -1;
 }
 
+struct alignas(16) ChunkMeta {
+  uint64_t seg_mask;   // bit l (l >= 1): slot l starts a new segment
+  uint64_t bb_mask;    // regular slots with a missing link
+  uint64_t irr_mask;   // slots with an explicit pull row
+  uint64_t nw_mask;    // near-wall slots
+  int32_t seg_first;   // segment of slot 0 of the chunk
+  int32_t bb_base;     // bbm index of the chunk's first bounce-back slot
+  int32_t irr_base;    // row of the chunk's first irregular slot
+  int32_t nw_base;     // near-wall index of the chunk's first near-wall slot
+};
+static_assert(sizeof(ChunkMeta) == 48, "ChunkMeta layout");
+
+__host__ __device__ inline int32_t popcount64(uint64_t m) { return __builtin_popcountll(m); }
+
+// Number of set bits of m below lane l (0 <= l < 64).
+__host__ __device__ inline int32_t rank_below(uint64_t m, int l) {
+  return l ? popcount64(m & (~uint64_t(0) >> (64 - l))) : 0;
+}
+
+// Segment of lane l in a chunk: the chunk's first segment plus the segment
+// starts at lanes 1..l.
+__host__ __device__ inline int32_t segment_of(const ChunkMeta& m, int l) {
+  return m.seg_first + popcount64(m.seg_mask & (~uint64_t(1)) & (~uint64_t(0) >> (63 - l)));
+}
+
 struct ColArgs {
   const void* fs_in;
   void* fs_out;
@@ -50,18 +78,17 @@ struct ColArgs {
   int32_t xcd_remap;
   int32_t xcd_first[8];
   int32_t xcd_offset[8];
-  const uint16_t* idx16;  // NCOL * npad; idx16[0] == IRREGULAR marks an irregular slot,
-                          // whose row index is idx16[1] | idx16[2] << 16
-  const int32_t* cbase;   // NCOL * nchunk
-  int32_t nchunk;
+  const ChunkMeta* meta;     // npad / CHUNK
+  const int32_t* seg_delta;  // NCOL per segment (32-byte aligned rows)
+  const uint32_t* bbm;       // missing-link masks (bit q), bb_mask order
   int32_t n_irr;
   const int32_t* irr_src;    // Q * n_irr pull sources (slot or -1 = bounce back)
-  const uint32_t* irr_kind;  // n_irr: NodeKind | ROW_NEARWALL
+  const uint32_t* irr_kind;  // n_irr: NodeKind
   const int32_t* irr_bcn;    // n_irr: slot behind an inlet/outlet, or -1
   const int32_t* irr_inlet;  // n_irr: inlet basis index, or -1
-  const double* irr_normal;  // 3 * n_irr (near-wall rows)
-  double* irr_rho_u;         // 4 * n_irr: rho (or rho - 1), u of the last collision (near-wall rows)
-  double* irr_wss;           // 5 * n_irr: |tau|, acc tau x/y/z, acc |tau| (near-wall rows)
+  int32_t n_nw;
+  const double* nw_normal;   // 3 * n_nw (SoA), near-wall slot order
+  double* nw_wss;            // 5 * n_nw: |tau|, acc tau x/y/z, acc |tau|
   double wss_C;
   int32_t wss_on;
   const int32_t* bslots;     // inlet/outlet slots
@@ -76,14 +103,15 @@ struct ColArgs {
   double outlet_rho;
   double omega;
   double force[3];
+  int32_t stream_stores;  // nontemporal population stores
   const float* porosity;  // per slot, or null
   const double* f_direct; // init input (FP64 unshifted, q * npad + slot), null = w_q
   double* f_out;          // materialize output
 };
 
 hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st);
-// Sum |tau|^2 and max |tau| over near-wall rows (2 doubles into out2);
-// partials needs 2 * ceil(n_irr / 256) doubles.
+// Sum |tau|^2 and max |tau| over near-wall nodes (2 doubles into out2);
+// partials needs 2 * ceil(n_nw / 256) doubles.
 hipError_t launch_columns_wss_stats(const ColArgs& a, double* partials, double* out2, hipStream_t st);
 
 }  // namespace lbm

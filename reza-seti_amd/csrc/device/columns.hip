@@ -1,18 +1,20 @@
-// Column-offset kernel family (see columns.h for the layout).
+// Column-segment kernel family (see columns.h for the layout).
 //
 // One launch per reference iteration:
 //   leading blocks  inlet/outlet nodes: pull, inlet/outlet rule, collide,
 //                   store (a short dependent-load chain; dispatched first so
 //                   its latency hides under the rest of the launch);
-//   other blocks    every other slot: pull (regular slots from 8 column bases
-//                   + 16-bit offsets, irregular rows from their int32 sources
-//                   with half-way bounce-back on missing links), collide,
-//                   store; near-wall rows also accumulate the wall shear
+//   other blocks    every other slot: pull (regular slots from their
+//                   segment's 8 lateral deltas, explicit rows from their int32
+//                   sources; half-way bounce-back on missing links), collide,
+//                   store. Near-wall nodes also accumulate the wall shear
 //                   stress (Simulation::ComputeWallShearStress,
 //                   Simulation.cpp:767-799) from the pulled populations and
-//                   the density/velocity they stored at the previous collision.
+//                   the equilibrium of the collision that produced them.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "../lattice.h"
 #include "collision.h"
@@ -32,27 +34,22 @@ struct ColKern {
   static constexpr int Q = L::Q;
   static constexpr bool SHIFT = P::shifted;
 
-  // Irregular row index of slot p, or -1.
-  __device__ static int32_t irregular_row(const ColArgs& a, int32_t p) {
-    if (a.idx16[p] != IRREGULAR) return -1;
-    return int32_t(uint32_t(a.idx16[a.npad + p]) | (uint32_t(a.idx16[2 * a.npad + p]) << 16));
-  }
-
-  // Pull of a regular slot; `chunk` is wave-uniform in the main blocks
-  // (scalar base loads).
-  __device__ static void pull_regular(const ColArgs& a, int32_t p, int32_t chunk, C* F) {
-    int32_t col[NCOL];
-#pragma unroll
-    for (int o = 0; o < NCOL; ++o) col[o] = a.cbase[o * a.nchunk + chunk] + int32_t(a.idx16[o * a.npad + p]);
+  // Pull of a regular slot p in segment `seg`; bit q of `bm` marks a missing
+  // link (half-way bounce-back: the node's own population opp(q)).
+  __device__ static void pull_regular(const ColArgs& a, int32_t p, int32_t seg, uint32_t bm, C* F) {
+    const int4* d4 = reinterpret_cast<const int4*>(a.seg_delta) + 2 * int64_t(seg);
+    const int4 d0 = d4[0], d1 = d4[1];
+    const int32_t d[NCOL] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int o = col_of(L::cx[q], L::cy[q]);
-      const int32_t s = (o < 0 ? p : col[o]) - L::cz[q];
-      F[q] = P::ld(a.fs_in, int64_t(q) * a.npad + s);
+      const int32_t s = (o < 0 ? p : p + d[o]) - L::cz[q];
+      const bool bb = (bm >> q) & 1u;
+      F[q] = P::ld(a.fs_in, int64_t(bb ? L::opp[q] : q) * a.npad + (bb ? p : s));
     }
   }
 
-  // Pull of an irregular row: explicit sources, bounce back on missing links.
+  // Pull of an explicit row.
   __device__ static void pull_irregular(const ColArgs& a, int32_t p, int32_t row, C* F) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -63,10 +60,16 @@ struct ColKern {
     }
   }
 
+  // Pull of any slot, metadata read per lane.
   __device__ static void pull_any(const ColArgs& a, int32_t p, C* F) {
-    const int32_t row = irregular_row(a, p);
-    if (row >= 0) pull_irregular(a, p, row, F);
-    else pull_regular(a, p, p / CHUNK, F);
+    const ChunkMeta m = a.meta[p / CHUNK];
+    const int l = p & (CHUNK - 1);
+    if ((m.irr_mask >> l) & 1u) {
+      pull_irregular(a, p, m.irr_base + rank_below(m.irr_mask, l), F);
+    } else {
+      const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
+      pull_regular(a, p, segment_of(m, l), bm, F);
+    }
   }
 
   __device__ static void load_direct(const ColArgs& a, int32_t p, C* F) {
@@ -82,38 +85,72 @@ struct ColKern {
     for (int q = 0; q < Q; ++q) a.f_out[int64_t(q) * a.npad + p] = SHIFT ? double(F[q]) + L::w[q] : double(F[q]);
   }
 
-  // Collide and store; a near-wall row keeps its density/velocity for the
-  // next step's stress.
   template <bool FORCE>
-  __device__ static void collide_store(const ColArgs& a, int32_t p, C* F, int32_t nw_row) {
+  __device__ static void collide_store(const ColArgs& a, int32_t p, C* F) {
     C drho, ux, uy, uz;
     const C frc[3] = {C(a.force[0]), C(a.force[1]), C(a.force[2])};
     const C alpha = (FORCE && a.porosity) ? C(a.porosity[p]) : C(0);
     collide_ext<L, C, SHIFT, FORCE>(F, C(a.omega), frc, alpha, false, drho, ux, uy, uz);
-    if (nw_row >= 0) {
-      const int64_t m = a.n_irr;
-      a.irr_rho_u[nw_row] = double(drho);
-      a.irr_rho_u[m + nw_row] = double(ux);
-      a.irr_rho_u[2 * m + nw_row] = double(uy);
-      a.irr_rho_u[3 * m + nw_row] = double(uz);
-    }
+    // Nontemporal stores when both population buffers together exceed the
+    // Infinity Cache: nothing written here is read again before the next step.
+    if (a.stream_stores) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) P::st(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+      for (int q = 0; q < Q; ++q) P::st_nt(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) P::st(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+    }
   }
 
-  // tau_i = C sum_a (f_a - feq_a) (c_a.n) (c_ai - (c_a.n) n_i), the reference
-  // expression (Simulation.cpp:775-797) with its inner sums written out.
-  __device__ static void wss_row(const ColArgs& a, int32_t row, const C* F) {
-    const int64_t m = a.n_irr;
-    const double n[3] = {a.irr_normal[row], a.irr_normal[m + row], a.irr_normal[2 * m + row]};
-    const C drho = C(a.irr_rho_u[row]), ux = C(a.irr_rho_u[m + row]), uy = C(a.irr_rho_u[2 * m + row]),
-            uz = C(a.irr_rho_u[3 * m + row]);
-    const C rho = SHIFT ? C(1) + drho : drho;
+  // Wall shear stress of near-wall node i from its pulled,
+  // boundary-treated populations F:
+  //   tau_i = C sum_a (f_a - feq_a) (c_a.n) (c_ai - (c_a.n) n_i)
+  // (Simulation.cpp:775-797 with its inner sums written out). feq is the
+  // equilibrium of the collision that produced this step's input; density
+  // and velocity are recovered from the node's own post-collision
+  // populations, which BGK and the Guo source leave with the same density
+  // and with momentum rho u (1 - alpha/2) + F/2.
+  struct Eq {
+    C s, rho, ux, uy, uz;
+  };
+
+  // Density and velocity of the collision that produced slot p's input
+  // (read before the pull so that only five values stay live across it).
+  template <bool FORCE>
+  __device__ static Eq wss_equilibrium(const ColArgs& a, int32_t p) {
+    C s = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const C v = P::ld(a.fs_in, int64_t(q) * a.npad + p);
+      s += v;
+      if (L::cx[q] > 0) jx += v;
+      if (L::cx[q] < 0) jx -= v;
+      if (L::cy[q] > 0) jy += v;
+      if (L::cy[q] < 0) jy -= v;
+      if (L::cz[q] > 0) jz += v;
+      if (L::cz[q] < 0) jz -= v;
+    }
+    const C rho = SHIFT ? C(1) + s : s;
+    C den = rho;
+    if (FORCE) {
+      const C alpha = a.porosity ? C(a.porosity[p]) : C(0);
+      jx -= C(0.5) * C(a.force[0]);
+      jy -= C(0.5) * C(a.force[1]);
+      jz -= C(0.5) * C(a.force[2]);
+      den = rho * (C(1) - C(0.5) * alpha);
+    }
+    return Eq{s, rho, jx / den, jy / den, jz / den};
+  }
+
+  __device__ static void wss_node(const ColArgs& a, int32_t i, const Eq& e, const C* F) {
+    const C ux = e.ux, uy = e.uy, uz = e.uz, rho = e.rho, s = e.s;
     const C usq = ux * ux + uy * uy + uz * uz;
+    const int64_t m = a.n_nw;
+    const double n[3] = {a.nw_normal[i], a.nw_normal[m + i], a.nw_normal[2 * m + i]};
     double tau[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int al = 0; al < Q; ++al) {
-      const double fne = double(F[al] - geq_of<L, C, SHIFT>(al, SHIFT ? drho : C(0), rho, ux, uy, uz, usq));
+      const double fne = double(F[al] - geq_of<L, C, SHIFT>(al, SHIFT ? s : C(0), rho, ux, uy, uz, usq));
       const double cn = L::cx[al] * n[0] + L::cy[al] * n[1] + L::cz[al] * n[2];
       tau[0] += fne * cn * (L::cx[al] - cn * n[0]);
       tau[1] += fne * cn * (L::cy[al] - cn * n[1]);
@@ -121,39 +158,41 @@ struct ColKern {
     }
     const double t0 = a.wss_C * tau[0], t1 = a.wss_C * tau[1], t2 = a.wss_C * tau[2];
     const double W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
-    a.irr_wss[row] = W;
-    a.irr_wss[m + row] += t0;
-    a.irr_wss[2 * m + row] += t1;
-    a.irr_wss[3 * m + row] += t2;
-    a.irr_wss[4 * m + row] += W;
+    a.nw_wss[i] = W;
+    a.nw_wss[m + i] += t0;
+    a.nw_wss[2 * m + i] += t1;
+    a.nw_wss[3 * m + i] += t2;
+    a.nw_wss[4 * m + i] += W;
   }
 
   __device__ static bool is_boundary_kind(const ColArgs& a, uint32_t kind) {
     return (kind == 2 && a.inlet_mode != 2) || (kind == 3 && a.outlet_mode != 2);
   }
 
+  // Slot p = lane l of a chunk whose metadata m is wave-uniform.
   template <int MODE, bool FORCE>
-  __device__ static void main_slot(const ColArgs& a, int32_t p, int32_t chunk) {
+  __device__ static void main_slot(const ColArgs& a, int32_t p, int l, const ChunkMeta& m) {
     C F[Q];
-    const int32_t row = irregular_row(a, p);
-    uint32_t kf = 0;
-    if (row >= 0) {
-      kf = a.irr_kind[row];
-      if (MODE != KMODE_INIT && is_boundary_kind(a, kf & ROW_KIND_MASK)) return;  // leading blocks own it
-    }
+    const bool irr = (m.irr_mask >> l) & 1u;
+    const int32_t row = irr ? m.irr_base + rank_below(m.irr_mask, l) : -1;
+    if (MODE != KMODE_INIT && irr && is_boundary_kind(a, a.irr_kind[row] & ROW_KIND_MASK)) return;
+    const bool nw = MODE == KMODE_STEP && a.wss_on && ((m.nw_mask >> l) & 1u);
+    Eq e{};
+    if (nw) e = wss_equilibrium<FORCE>(a, p);
     if (MODE == KMODE_INIT) {
       load_direct(a, p, F);
-    } else if (row >= 0) {
+    } else if (irr) {
       pull_irregular(a, p, row, F);
-      if (MODE == KMODE_STEP && (kf & ROW_NEARWALL) && a.wss_on) wss_row(a, row, F);
     } else {
-      pull_regular(a, p, chunk, F);
+      const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
+      pull_regular(a, p, segment_of(m, l), bm, F);
     }
+    if (nw) wss_node(a, m.nw_base + rank_below(m.nw_mask, l), e, F);
     if (MODE == KMODE_MATERIALIZE) {
       write_direct(a, p, F);
       return;
     }
-    collide_store<FORCE>(a, p, F, (row >= 0 && (kf & ROW_NEARWALL)) ? row : -1);
+    collide_store<FORCE>(a, p, F);
   }
 
   __device__ static C inlet_speed(const ColArgs& a, int32_t i) {
@@ -185,14 +224,18 @@ struct ColKern {
              (G[q] - geq_of<L, C, SHIFT>(q, dg, rg, gx, gy, gz, usq_g));
   }
 
-  // Inlet/outlet node p (always an irregular row).
+  // Inlet/outlet node p (always an explicit row).
   template <int MODE, bool FORCE>
   __device__ static void boundary_slot(const ColArgs& a, int32_t p) {
-    const int32_t row = irregular_row(a, p);
-    if (row < 0) return;
-    const uint32_t kf = a.irr_kind[row];
-    const uint32_t kind = kf & ROW_KIND_MASK;
+    const ChunkMeta m = a.meta[p / CHUNK];
+    const int l = p & (CHUNK - 1);
+    if (!((m.irr_mask >> l) & 1u)) return;
+    const int32_t row = m.irr_base + rank_below(m.irr_mask, l);
+    const uint32_t kind = a.irr_kind[row] & ROW_KIND_MASK;
     const int32_t u = a.irr_bcn[row];
+    const bool nw = MODE == KMODE_STEP && a.wss_on && ((m.nw_mask >> l) & 1u);
+    Eq e{};
+    if (nw) e = wss_equilibrium<FORCE>(a, p);
     C F[Q];
     if (kind == 2) {
       if (a.inlet_mode == INLET_NEE && u >= 0) {
@@ -211,9 +254,13 @@ struct ColKern {
       if (a.outlet_mode == OUTLET_REF) {
         // Simulation::ApplyOutletBc: every population of the node behind,
         // after that node's own inlet rule (Zou-He runs first).
-        const int32_t urow = irregular_row(a, u);
-        if (urow >= 0 && (a.irr_kind[urow] & ROW_KIND_MASK) == 2 && a.inlet_mode == INLET_ZOUHE) {
-          if constexpr (Q == 19) zouhe_ref<C, SHIFT>(F, inlet_speed(a, a.irr_inlet[urow]), a.inlet_normal);
+        const ChunkMeta mu = a.meta[u / CHUNK];
+        const int lu = u & (CHUNK - 1);
+        if (((mu.irr_mask >> lu) & 1u) && a.inlet_mode == INLET_ZOUHE) {
+          const int32_t urow = mu.irr_base + rank_below(mu.irr_mask, lu);
+          if ((a.irr_kind[urow] & ROW_KIND_MASK) == 2) {
+            if constexpr (Q == 19) zouhe_ref<C, SHIFT>(F, inlet_speed(a, a.irr_inlet[urow]), a.inlet_normal);
+          }
         }
       } else {
         nee_inplace(F, C(a.outlet_rho), C(0), C(0), C(0), true);
@@ -222,17 +269,17 @@ struct ColKern {
       pull_irregular(a, p, row, F);
     }
     // The stress follows every boundary rule, as in the reference order.
-    if (MODE == KMODE_STEP && (kf & ROW_NEARWALL) && a.wss_on) wss_row(a, row, F);
+    if (nw) wss_node(a, m.nw_base + rank_below(m.nw_mask, l), e, F);
     if (MODE == KMODE_MATERIALIZE) {
       write_direct(a, p, F);
       return;
     }
-    collide_store<FORCE>(a, p, F, (kf & ROW_NEARWALL) ? row : -1);
+    collide_store<FORCE>(a, p, F);
   }
 };
 
 template <class L, class S, int MODE, bool FORCE>
-__global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
+__device__ inline void columns_body(const ColArgs& a) {
   const int32_t b = blockIdx.x;
   if (b < a.b_blocks) {
     const int32_t i = b * BLOCK + threadIdx.x;
@@ -245,16 +292,37 @@ __global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
     lb = a.xcd_offset[x] + (b - a.xcd_first[x]) / 8;
   }
   const int32_t p = lb * BLOCK + threadIdx.x;
-  // One wavefront covers one 64-slot chunk: its base loads are scalar.
+  // One wavefront covers one chunk: its metadata is wave-uniform.
   const int32_t chunk = __builtin_amdgcn_readfirstlane(p / CHUNK);
-  if (p < a.n) ColKern<L, S>::template main_slot<MODE, FORCE>(a, p, chunk);
+  const ChunkMeta m = a.meta[chunk];
+  if (p < a.n) ColKern<L, S>::template main_slot<MODE, FORCE>(a, p, p & (CHUNK - 1), m);
 }
+
+template <class L, class S, int MODE, bool FORCE>
+__global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
+  columns_body<L, S, MODE, FORCE>(a);
+}
+
+// Occupancy-bounded variant for the float-arithmetic precisions: the
+// inlet/outlet path needs more registers than the main path, and a
+// waves-per-EU bound keeps the main path's occupancy while that rare path
+// spills instead. (FP64 keeps the unbounded kernel: its main path spills.)
+#ifndef LBM_COLUMNS_WAVES_PER_EU
+#define LBM_COLUMNS_WAVES_PER_EU 7
+#endif
+#if LBM_COLUMNS_WAVES_PER_EU > 0
+template <class L, class S, int MODE, bool FORCE>
+__global__ __attribute__((amdgpu_waves_per_eu(LBM_COLUMNS_WAVES_PER_EU))) __launch_bounds__(BLOCK) void
+columns_kernel_occ(ColArgs a) {
+  columns_body<L, S, MODE, FORCE>(a);
+}
+#endif
 
 __global__ __launch_bounds__(BLOCK) void wss_stats_cols(ColArgs a, double* partials) {
   const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
   double s = 0.0, mx = 0.0;
-  if (i < a.n_irr && (a.irr_kind[i] & ROW_NEARWALL)) {
-    const double w = a.irr_wss[i];
+  if (i < a.n_nw) {
+    const double w = a.nw_wss[i];
     s = w * w;
     mx = w;
   }
@@ -284,7 +352,14 @@ __global__ __launch_bounds__(BLOCK) void wss_stats_cols(ColArgs a, double* parti
 template <class L, class S, int MODE, bool FORCE>
 hipError_t launch_one(const ColArgs& a, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>(a.b_blocks + a.m_blocks);
-  if (grid) hipLaunchKernelGGL((columns_kernel<L, S, MODE, FORCE>), dim3(grid), dim3(BLOCK), 0, st, a);
+  if (!grid) return hipSuccess;
+#if LBM_COLUMNS_WAVES_PER_EU > 0
+  if constexpr (!std::is_same<S, double>::value) {
+    hipLaunchKernelGGL((columns_kernel_occ<L, S, MODE, FORCE>), dim3(grid), dim3(BLOCK), 0, st, a);
+    return hipGetLastError();
+  }
+#endif
+  hipLaunchKernelGGL((columns_kernel<L, S, MODE, FORCE>), dim3(grid), dim3(BLOCK), 0, st, a);
   return hipGetLastError();
 }
 
@@ -320,7 +395,7 @@ hipError_t launch_columns(int Q, int precision, int kmode, bool force, const Col
 }
 
 hipError_t launch_columns_wss_stats(const ColArgs& a, double* partials, double* out2, hipStream_t st) {
-  const int32_t nb = (a.n_irr + BLOCK - 1) / BLOCK;
+  const int32_t nb = (a.n_nw + BLOCK - 1) / BLOCK;
   if (nb == 0) return hipMemsetAsync(out2, 0, 2 * sizeof(double), st);
   hipLaunchKernelGGL(wss_stats_cols, dim3(nb), dim3(BLOCK), 0, st, a, partials);
   hipError_t e = hipGetLastError();

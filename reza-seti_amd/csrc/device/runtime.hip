@@ -105,20 +105,20 @@ struct lbm_solver {
   double last_coef[MAX_MODES] = {0};
   StepArgs base{};
 
-  // Column-offset family (half-way bounce-back walls), see columns.h.
+  // Column-segment family (half-way bounce-back walls), see columns.h.
   int family = 0;                    // 0: reference rules (kernels.hip), 1: columns.hip
-  uint16_t* idx16 = nullptr;
-  int32_t* cbase = nullptr;
+  ChunkMeta* meta = nullptr;
+  int32_t* seg_delta = nullptr;
+  uint32_t* bbm = nullptr;
   int32_t* irr_src = nullptr;
   uint32_t* irr_kind = nullptr;
   int32_t* irr_bcn = nullptr;
   int32_t* irr_inlet = nullptr;
   int32_t* bslots = nullptr;
-  double* irr_normal = nullptr;
-  double* irr_rho_u = nullptr;
-  double* irr_wss = nullptr;
-  int64_t n_irr = 0;
-  std::vector<int32_t> nw_rows;      // near-wall list order -> irregular row
+  double* nw_normal = nullptr;
+  double* nw_wss = nullptr;
+  int64_t n_irr = 0, n_nw = 0;
+  std::vector<int32_t> nw_rows;      // near-wall list order -> near-wall slot rank
   ColArgs cbase_args{};
 };
 
@@ -144,8 +144,8 @@ int dalloc_raw(lbm_solver* s, void** p, size_t bytes) {
 void free_all(lbm_solver* s) {
   void* ptrs[] = {s->fs[0], s->fs[1], s->fprev[0], s->fprev[1], s->src, s->sflags, s->bc_nbr, s->inlet_slot,
                   s->inlet_basis, s->normal, s->rho_u, s->wss, s->porosity, s->err, s->f_tmp, s->partials,
-                  s->red_out, s->idx16, s->cbase, s->irr_src, s->irr_kind, s->irr_bcn, s->irr_inlet,
-                  s->bslots, s->irr_normal, s->irr_rho_u, s->irr_wss};
+                  s->red_out, s->meta, s->seg_delta, s->bbm, s->irr_src, s->irr_kind, s->irr_bcn,
+                  s->irr_inlet, s->bslots, s->nw_normal, s->nw_wss};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -272,9 +272,9 @@ void xcd_tables(int32_t Bs, int32_t Bb, int32_t* first, int32_t* offset) {
   }
 }
 
-// Layout of the column-offset family (columns.h): slots = reference node
-// order; regular nodes store 8 lateral column slots as 16-bit offsets from a
-// per-64-slot base, irregular nodes an explicit int32 pull row.
+// Layout of the column-segment family (columns.h): slots = reference node
+// order; regular nodes pull through their segment's 8 lateral deltas,
+// irregular nodes through an explicit int32 row.
 int create_columns(lbm_solver* s, const Tables& t) {
   const int Q = t.Q;
   const int64_t N = t.N;
@@ -298,118 +298,166 @@ int create_columns(lbm_solver* s, const Tables& t) {
   auto srcq = [&](int q, int64_t n) -> int32_t {
     return q == L.rest ? static_cast<int32_t>(n) : t.nb[static_cast<size_t>(L.opp[q]) * N + n];
   };
-  int qcol[NCOL];
-  for (int o = 0; o < NCOL; ++o) {
-    qcol[o] = -1;
-    for (int q = 0; q < Q; ++q)
-      if (L.cz[q] == 0 && col_of(L.cx[q], L.cy[q]) == o) qcol[o] = q;
-    if (qcol[o] < 0) return fail(LBM_ERR_ARG, "lattice lacks an in-plane direction");
-  }
   const bool inl_b = p.inlet_mode != LBM_INLET_NONE, out_b = p.outlet_mode != LBM_OUTLET_NONE;
-  // Near-wall nodes keep their stress data in a row, so they are irregular.
-  std::vector<int32_t> nw_index(N, -1);
-  for (size_t i = 0; i < t.nearwall.size(); ++i) nw_index[t.nearwall[i]] = static_cast<int32_t>(i);
-  std::vector<uint8_t> reg(N, 0);
-  std::vector<int32_t> col(static_cast<size_t>(NCOL) * N, -1);
+
+  // Column segments: maximal runs of slots with the same (i, j) and
+  // consecutive k.
+  std::vector<int32_t> seg_of(N), seg_start;
   for (int64_t n = 0; n < N; ++n) {
-    for (int o = 0; o < NCOL; ++o) col[static_cast<size_t>(o) * N + n] = srcq(qcol[o], n);
-    bool r = !((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b)) && nw_index[n] < 0;
+    const int32_t* c = &t.ijk[3 * static_cast<size_t>(n)];
+    if (n == 0 || c[0] != c[-3] || c[1] != c[-2] || c[2] != c[-1] + 1) seg_start.push_back(static_cast<int32_t>(n));
+    seg_of[n] = static_cast<int32_t>(seg_start.size() - 1);
+  }
+  const int64_t nseg = static_cast<int64_t>(seg_start.size());
+  seg_start.push_back(static_cast<int32_t>(N));
+
+  // Per segment and lateral offset o, the delta src + c_z - n that most of
+  // its links agree on (periodic wraps and a neighbour column that changes
+  // segment disagree; their nodes take explicit rows).
+  std::vector<int32_t> delta(static_cast<size_t>(NCOL) * std::max<int64_t>(nseg, 1), 0);
+  {
+    std::vector<int32_t> cand;
+    for (int64_t sg = 0; sg < nseg; ++sg)
+      for (int o = 0; o < NCOL; ++o) {
+        cand.clear();
+        for (int64_t n = seg_start[sg]; n < seg_start[sg + 1]; ++n)
+          for (int q = 0; q < Q; ++q) {
+            if (col_of(L.cx[q], L.cy[q]) != o) continue;
+            const int32_t sq = srcq(q, n);
+            if (sq >= 0) cand.push_back(static_cast<int32_t>(int64_t(sq) + L.cz[q] - n));
+          }
+        if (cand.empty()) continue;
+        std::sort(cand.begin(), cand.end());
+        int32_t best = cand[0];
+        size_t best_run = 0;
+        for (size_t i = 0; i < cand.size();) {
+          size_t j = i;
+          while (j < cand.size() && cand[j] == cand[i]) ++j;
+          if (j - i > best_run) {
+            best_run = j - i;
+            best = cand[i];
+          }
+          i = j;
+        }
+        delta[static_cast<size_t>(sg) * NCOL + o] = best;
+      }
+  }
+
+  // Regular nodes: every existing link follows its segment's delta (or the
+  // own column); missing links become bounce-back bits.
+  std::vector<uint8_t> reg(N, 0);
+  std::vector<uint32_t> bbm_node(N, 0);
+  for (int64_t n = 0; n < N; ++n) {
+    bool r = !((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b));
+    uint32_t bm = 0;
     for (int q = 0; q < Q && r; ++q) {
       const int32_t sq = srcq(q, n);
+      if (sq < 0) {
+        bm |= 1u << q;
+        continue;
+      }
       const int o = col_of(L.cx[q], L.cy[q]);
-      const int64_t base = o < 0 ? n : col[static_cast<size_t>(o) * N + n];
-      if (sq < 0 || base < 0 || int64_t(sq) != base - L.cz[q]) r = false;
+      const int64_t base = o < 0 ? n : n + delta[static_cast<size_t>(seg_of[n]) * NCOL + o];
+      if (int64_t(sq) != base - L.cz[q]) r = false;
     }
     reg[n] = r ? 1 : 0;
+    bbm_node[n] = r ? bm : 0;
   }
-  const int64_t nchunk = npad / CHUNK;
-  std::vector<int32_t> cb(static_cast<size_t>(NCOL) * nchunk, 0);
-  std::vector<uint16_t> idx(static_cast<size_t>(NCOL) * npad, 0);
-  for (int64_t c = 0; c < nchunk; ++c) {
-    const int64_t a0 = c * CHUNK, a1 = std::min<int64_t>(N, a0 + CHUNK);
-    for (bool again = true; again;) {
-      again = false;
-      for (int o = 0; o < NCOL && !again; ++o) {
-        int64_t mn = INT64_MAX;
-        for (int64_t n = a0; n < a1; ++n)
-          if (reg[n]) mn = std::min<int64_t>(mn, col[static_cast<size_t>(o) * N + n]);
-        if (mn == INT64_MAX) mn = 0;
-        for (int64_t n = a0; n < a1; ++n)
-          if (reg[n] && col[static_cast<size_t>(o) * N + n] - mn > 0xFFFE) {
-            reg[n] = 0;  // too far from its chunk: give it an explicit row
-            again = true;
-          }
-        cb[static_cast<size_t>(o) * nchunk + c] = static_cast<int32_t>(mn);
-      }
-    }
-    for (int o = 0; o < NCOL; ++o)
-      for (int64_t n = a0; n < a1; ++n)
-        if (reg[n])
-          idx[static_cast<size_t>(o) * npad + n] =
-              static_cast<uint16_t>(col[static_cast<size_t>(o) * N + n] - cb[static_cast<size_t>(o) * nchunk + c]);
-  }
+
   std::vector<int32_t> inlet_index(N, -1);
   for (size_t i = 0; i < t.inlet_nodes.size(); ++i) inlet_index[t.inlet_nodes[i]] = static_cast<int32_t>(i);
-  std::vector<int32_t> rows;
-  for (int64_t n = 0; n < N; ++n)
-    if (!reg[n]) rows.push_back(static_cast<int32_t>(n));
+  std::vector<uint8_t> is_nw(N, 0);
+  for (int32_t n : t.nearwall) is_nw[n] = 1;
+
+  // Chunk metadata and the compact per-kind arrays, all in slot order.
+  const int64_t nchunk = npad / CHUNK;
+  std::vector<ChunkMeta> meta(nchunk);
+  std::vector<uint32_t> bbm;
+  std::vector<int32_t> rows, nw_slot_order, bsl;
+  std::vector<int32_t> nw_rank(N, -1);
+  for (int64_t c = 0; c < nchunk; ++c) {
+    ChunkMeta& m = meta[c];
+    std::memset(&m, 0, sizeof(m));
+    m.seg_first = c * CHUNK < N ? seg_of[c * CHUNK] : 0;
+    m.bb_base = static_cast<int32_t>(bbm.size());
+    m.irr_base = static_cast<int32_t>(rows.size());
+    m.nw_base = static_cast<int32_t>(nw_slot_order.size());
+    for (int l = 0; l < CHUNK; ++l) {
+      const int64_t n = c * CHUNK + l;
+      if (n >= N) break;
+      const uint64_t bit = uint64_t(1) << l;
+      if (l > 0 && seg_of[n] != seg_of[n - 1]) m.seg_mask |= bit;
+      if (!reg[n]) {
+        m.irr_mask |= bit;
+        rows.push_back(static_cast<int32_t>(n));
+        if ((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b))
+          bsl.push_back(static_cast<int32_t>(n));
+      } else if (bbm_node[n]) {
+        m.bb_mask |= bit;
+        bbm.push_back(bbm_node[n]);
+      }
+      if (is_nw[n]) {
+        m.nw_mask |= bit;
+        nw_rank[n] = static_cast<int32_t>(nw_slot_order.size());
+        nw_slot_order.push_back(static_cast<int32_t>(n));
+      }
+    }
+  }
   const int64_t nirr = static_cast<int64_t>(rows.size());
+  const int64_t nnw = static_cast<int64_t>(nw_slot_order.size());
   s->n_irr = nirr;
+  s->n_nw = nnw;
+  s->nw_rows.resize(t.nearwall.size());
+  for (size_t i = 0; i < t.nearwall.size(); ++i) s->nw_rows[i] = nw_rank[t.nearwall[i]];
   std::vector<int32_t> isrc(static_cast<size_t>(Q) * std::max<int64_t>(nirr, 1), -1);
   std::vector<uint32_t> ikind(std::max<int64_t>(nirr, 1), 0);
   std::vector<int32_t> ibcn(std::max<int64_t>(nirr, 1), -1), iinl(std::max<int64_t>(nirr, 1), -1);
-  std::vector<double> inrm(3 * static_cast<size_t>(std::max<int64_t>(nirr, 1)), 0.0);
-  std::vector<int32_t> bsl;
-  s->nw_rows.assign(t.nearwall.size(), -1);
   for (int64_t r = 0; r < nirr; ++r) {
     const int32_t n = rows[r];
     for (int q = 0; q < Q; ++q) isrc[static_cast<size_t>(q) * nirr + r] = srcq(q, n);
     ikind[r] = t.kind[n];
     ibcn[r] = t.bc_nbr[n];
     iinl[r] = inlet_index[n];
-    if (nw_index[n] >= 0) {
-      const int32_t i = nw_index[n];
-      ikind[r] |= ROW_NEARWALL;
-      s->nw_rows[i] = static_cast<int32_t>(r);
-      for (int d = 0; d < 3; ++d) inrm[static_cast<size_t>(d) * nirr + r] = t.normals[3 * static_cast<size_t>(i) + d];
-    }
-    idx[n] = IRREGULAR;
-    idx[static_cast<size_t>(npad) + n] = static_cast<uint16_t>(r & 0xFFFF);
-    idx[static_cast<size_t>(2) * npad + n] = static_cast<uint16_t>(uint32_t(r) >> 16);
-    if ((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b)) bsl.push_back(n);
   }
-  // Decode every regular slot the way the kernel does and compare with the
-  // int32 pull table.
+  std::vector<double> nrm(3 * static_cast<size_t>(std::max<int64_t>(nnw, 1)), 0.0);
+  for (size_t i = 0; i < t.nearwall.size(); ++i)
+    for (int d = 0; d < 3; ++d)
+      nrm[static_cast<size_t>(d) * nnw + nw_rank[t.nearwall[i]]] = t.normals[3 * i + d];
+
+  // Decode every slot the way the kernel does and compare with the int32
+  // pull table (-1 = bounce back).
   for (int64_t n = 0; n < N; ++n) {
-    if (!reg[n]) continue;
-    const int64_t c = n / CHUNK;
+    const ChunkMeta& m = meta[n / CHUNK];
+    const int l = static_cast<int>(n % CHUNK);
     for (int q = 0; q < Q; ++q) {
-      const int o = col_of(L.cx[q], L.cy[q]);
-      const int64_t colv = o < 0 ? n
-                                 : int64_t(cb[static_cast<size_t>(o) * nchunk + c]) +
-                                       idx[static_cast<size_t>(o) * npad + n];
-      if (colv - L.cz[q] != srcq(q, n))
-        return fail(LBM_ERR_STATE, "column-offset encoding does not reproduce the neighbour table");
+      int32_t got;
+      if ((m.irr_mask >> l) & 1u) {
+        got = isrc[static_cast<size_t>(q) * nirr + m.irr_base + rank_below(m.irr_mask, l)];
+      } else {
+        const uint32_t bm = ((m.bb_mask >> l) & 1u) ? bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
+        const int o = col_of(L.cx[q], L.cy[q]);
+        const int64_t base = o < 0 ? n : n + delta[static_cast<size_t>(segment_of(m, l)) * NCOL + o];
+        got = ((bm >> q) & 1u) ? -1 : static_cast<int32_t>(base - L.cz[q]);
+      }
+      if (got != srcq(q, n)) return fail(LBM_ERR_STATE, "column-segment encoding does not reproduce the neighbour table");
     }
-    if (idx[n] == IRREGULAR) return fail(LBM_ERR_STATE, "regular slot carries the irregular marker");
+    if (((m.nw_mask >> l) & 1u) != is_nw[n]) return fail(LBM_ERR_STATE, "near-wall mask mismatch");
   }
-  for (int32_t r : s->nw_rows)
-    if (r < 0) return fail(LBM_ERR_STATE, "near-wall node without a row");
 
   int rc = LBM_OK;
   const size_t pop_bytes = static_cast<size_t>(Q) * npad * s->esize;
   const size_t ninl = std::max<size_t>(t.inlet_nodes.size(), 1);
-  const size_t nrow = static_cast<size_t>(std::max<int64_t>(nirr, 1));
+  const size_t nnw1 = static_cast<size_t>(std::max<int64_t>(nnw, 1));
   const int64_t nblk = npad / BLOCK;
   if ((rc = dalloc_raw(s, &s->fs[0], pop_bytes)) || (rc = dalloc_raw(s, &s->fs[1], pop_bytes)) ||
-      (rc = dalloc(s, &s->idx16, idx.size())) || (rc = dalloc(s, &s->cbase, cb.size())) ||
+      (rc = dalloc(s, &s->meta, meta.size())) || (rc = dalloc(s, &s->seg_delta, delta.size())) ||
+      (rc = dalloc(s, &s->bbm, std::max<size_t>(bbm.size(), 1))) ||
       (rc = dalloc(s, &s->irr_src, isrc.size())) || (rc = dalloc(s, &s->irr_kind, ikind.size())) ||
       (rc = dalloc(s, &s->irr_bcn, ibcn.size())) || (rc = dalloc(s, &s->irr_inlet, iinl.size())) ||
       (rc = dalloc(s, &s->bslots, std::max<size_t>(bsl.size(), 1))) ||
-      (rc = dalloc(s, &s->irr_normal, 3 * nrow)) || (rc = dalloc(s, &s->irr_rho_u, 4 * nrow)) ||
-      (rc = dalloc(s, &s->irr_wss, 5 * nrow)) || (rc = dalloc(s, &s->err, 1)) ||
-      (rc = dalloc(s, &s->inlet_basis, MAX_MODES * ninl)) ||
-      (rc = dalloc(s, &s->partials, static_cast<size_t>(4 * nblk + 2 * ((nrow + BLOCK - 1) / BLOCK) + 8))) ||
+      (rc = dalloc(s, &s->nw_normal, 3 * nnw1)) || (rc = dalloc(s, &s->nw_wss, 5 * nnw1)) ||
+      (rc = dalloc(s, &s->err, 1)) || (rc = dalloc(s, &s->inlet_basis, MAX_MODES * ninl)) ||
+      (rc = dalloc(s, &s->partials, static_cast<size_t>(4 * nblk + 2 * ((nnw1 + BLOCK - 1) / BLOCK) + 8))) ||
       (rc = dalloc(s, &s->red_out, 8)))
     return rc;
   std::vector<double> basis(MAX_MODES * ninl, 0.0);
@@ -417,13 +465,13 @@ int create_columns(lbm_solver* s, const Tables& t) {
   auto up = [&](void* d, const void* h, size_t bytes) {
     return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s->stream) : hipSuccess;
   };
-  if (up(s->idx16, idx.data(), idx.size() * 2) || up(s->cbase, cb.data(), cb.size() * 4) ||
+  if (up(s->meta, meta.data(), meta.size() * sizeof(ChunkMeta)) || up(s->seg_delta, delta.data(), delta.size() * 4) ||
+      up(s->bbm, bbm.data(), bbm.size() * 4) ||
       up(s->irr_src, isrc.data(), isrc.size() * 4) || up(s->irr_kind, ikind.data(), ikind.size() * 4) ||
       up(s->irr_bcn, ibcn.data(), ibcn.size() * 4) || up(s->irr_inlet, iinl.data(), iinl.size() * 4) ||
-      up(s->bslots, bsl.data(), bsl.size() * 4) || up(s->irr_normal, inrm.data(), inrm.size() * 8) ||
+      up(s->bslots, bsl.data(), bsl.size() * 4) || up(s->nw_normal, nrm.data(), nrm.size() * 8) ||
       up(s->inlet_basis, basis.data(), basis.size() * 8) ||
-      hipMemsetAsync(s->irr_wss, 0, 5 * nrow * 8, s->stream) ||
-      hipMemsetAsync(s->irr_rho_u, 0, 4 * nrow * 8, s->stream) ||
+      hipMemsetAsync(s->nw_wss, 0, 5 * nnw1 * 8, s->stream) ||
       hipMemsetAsync(s->err, 0, 4, s->stream) || hipMemsetAsync(s->fs[0], 0, pop_bytes, s->stream) ||
       hipMemsetAsync(s->fs[1], 0, pop_bytes, s->stream))
     return fail(LBM_ERR_HIP, "upload failed");
@@ -441,17 +489,17 @@ int create_columns(lbm_solver* s, const Tables& t) {
   c.m_blocks = static_cast<int32_t>(nblk);
   c.xcd_remap = 1;
   xcd_tables(c.b_blocks, c.m_blocks, c.xcd_first, c.xcd_offset);
-  c.idx16 = s->idx16;
-  c.cbase = s->cbase;
-  c.nchunk = static_cast<int32_t>(nchunk);
+  c.meta = s->meta;
+  c.seg_delta = s->seg_delta;
+  c.bbm = s->bbm;
   c.n_irr = static_cast<int32_t>(nirr);
   c.irr_src = s->irr_src;
   c.irr_kind = s->irr_kind;
   c.irr_bcn = s->irr_bcn;
   c.irr_inlet = s->irr_inlet;
-  c.irr_normal = s->irr_normal;
-  c.irr_rho_u = s->irr_rho_u;
-  c.irr_wss = s->irr_wss;
+  c.n_nw = static_cast<int32_t>(nnw);
+  c.nw_normal = s->nw_normal;
+  c.nw_wss = s->nw_wss;
   c.wss_C = wss_C;
   c.bslots = s->bslots;
   c.n_bslots = static_cast<int32_t>(bsl.size());
@@ -464,6 +512,8 @@ int create_columns(lbm_solver* s, const Tables& t) {
   c.outlet_rho = p.outlet_rho;
   c.omega = omega;
   for (int i = 0; i < 3; ++i) c.force[i] = p.force[i];
+  // Both buffers larger than the 256 MiB Infinity Cache: stream the stores.
+  c.stream_stores = 2 * pop_bytes > (size_t(256) << 20) ? 1 : 0;
 
   // Generic fields used by the shared moments kernel and init_state.
   StepArgs& a = s->base;
@@ -839,10 +889,10 @@ int lbm_get_wss(lbm_solver* s, double* wss, double* acc_tau, double* acc_mag) {
   if (!s) return fail(LBM_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->prm.device));
   if (s->family == 1) {
-    const int64_t m = s->n_irr;
+    const int64_t m = s->n_nw;
     std::vector<double> h(static_cast<size_t>(5 * m));
     if (m) {
-      HIP_TRY(hipMemcpyAsync(h.data(), s->irr_wss, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+      HIP_TRY(hipMemcpyAsync(h.data(), s->nw_wss, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
       HIP_TRY(hipStreamSynchronize(s->stream));
     }
     for (size_t i = 0; i < s->nw_rows.size(); ++i) {

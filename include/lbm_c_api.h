@@ -204,7 +204,8 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out);
  * lbm_step inlet speed (the reference's uniform plug). n_modes <= 16. */
 int lbm_set_inlet_profile(lbm_solver* s, int32_t n_modes, const double* basis);
 
-/* Extension: Brinkman damping field (>= 0) per node; NULL clears it. */
+/* Extension: Brinkman damping field (>= 0) per node; NULL clears it. Before
+ * the first step the initial collision is redone with the new field. */
 int lbm_set_porosity(lbm_solver* s, const float* alpha);
 
 /* Advance nsteps reference iterations (Simulation::Iterate loop body,
@@ -280,6 +281,38 @@ int lbm_get_solver_info(const lbm_solver* s, lbm_solver_info* out);
  * and a launch of known size for calibrating traffic counters
  * (kernel copy_probe_kernel). No reference counterpart. */
 int lbm_copy_probe(int32_t device, int64_t bytes, int32_t iters, double* gbps);
+
+/* ------------------------------------------------------------- adjoint
+ * No counterpart in the reference (it has no differentiation). Discrete
+ * adjoint of the half-way bounce-back step (LBM_WALL_HALFWAY_BB) with NEE or
+ * no inlet/outlet rule, for the objective
+ *   J = (1/T) sum_{t=1..T} [ wss_weight * sum_{x in segment} |tau_t(x)|^2
+ *                            + dp_weight * (rho_in(t) - rho_out(t)) / 3 ]
+ * where tau is the wall shear stress of Simulation::ComputeWallShearStress
+ * (Simulation.cpp:767-799) and rho_in / rho_out are the mean densities of the
+ * inlet / outlet nodes after their boundary rule. */
+typedef struct {
+  double wss_weight;
+  double dp_weight;
+  const uint8_t* wss_mask; /* per near-wall node (lbm_geometry_get_nearwall order), 1 = in the segment; NULL = all */
+} lbm_objective;
+
+/* Forward pass of nsteps from the state set by lbm_create / lbm_set_populations,
+ * with inlet coefficients coef[t * n_modes + m] (NULL: the lbm_step inlet
+ * schedule). Stores the state every checkpoint_every steps and returns J. */
+int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const lbm_objective* obj,
+                        int32_t checkpoint_every, double* J);
+/* Reverse pass of the last lbm_adjoint_forward: dJ/dcoef[t * n_modes + m]
+ * and dJ/dalpha per node (either may be NULL). Inside each checkpoint
+ * interval the states are recomputed on a binomial (revolve) schedule with
+ * `slots` stored states (<= 0: as many as fit, at most checkpoint_every - 1).
+ * The solver state afterwards is the forward pass's final state. */
+int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double* grad_porosity);
+/* The binomial schedule used inside one interval of nsteps steps with
+ * `slots` buffers: 5 int32 per action (op 0 advance / 1 reverse, from, to,
+ * buffer from, buffer to; buffer -1 = the interval's stored start state,
+ * -2 = scratch). actions may be NULL to query the count. */
+int lbm_checkpoint_schedule(int32_t nsteps, int32_t slots, int32_t* actions, int64_t capacity, int64_t* count);
 
 const char* lbm_last_error(void);
 

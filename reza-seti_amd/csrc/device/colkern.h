@@ -1,0 +1,280 @@
+// Per-node building blocks of the column-segment step kernel (pull with
+// half-way bounce-back, non-equilibrium extrapolation, wall shear stress,
+// collision), shared by the forward kernels (columns.hip) and their adjoint
+// (adjoint.hip). Layout: columns.h.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "../lattice.h"
+#include "collision.h"
+#include "columns.h"
+#include "kernels.h"
+
+namespace lbm {
+namespace colk {
+
+using namespace dev;
+
+template <class L, class S>
+struct ColKern {
+  using P = Pop<S>;
+  using C = typename P::C;
+  static constexpr int Q = L::Q;
+  static constexpr bool SHIFT = P::shifted;
+
+  // Pull of a regular slot p in segment `seg`; bit q of `bm` marks a missing
+  // link (half-way bounce-back: the node's own population opp(q)).
+  __device__ static void pull_regular(const ColArgs& a, int32_t p, int32_t seg, uint32_t bm, C* F) {
+    const int4* d4 = reinterpret_cast<const int4*>(a.seg_delta) + 2 * int64_t(seg);
+    const int4 d0 = d4[0], d1 = d4[1];
+    const int32_t d[NCOL] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int o = col_of(L::cx[q], L::cy[q]);
+      const int32_t s = (o < 0 ? p : p + d[o]) - L::cz[q];
+      const bool bb = (bm >> q) & 1u;
+      F[q] = P::ld(a.fs_in, int64_t(bb ? L::opp[q] : q) * a.npad + (bb ? p : s));
+    }
+  }
+
+  // Pull of an explicit row.
+  __device__ static void pull_irregular(const ColArgs& a, int32_t p, int32_t row, C* F) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int32_t s = a.irr_src[int64_t(q) * a.n_irr + row];
+      const int32_t slot = s >= 0 ? s : p;
+      const int qq = s >= 0 ? q : L::opp[q];
+      F[q] = P::ld(a.fs_in, int64_t(qq) * a.npad + slot);
+    }
+  }
+
+  // Pull of any slot, metadata read per lane.
+  __device__ static void pull_any(const ColArgs& a, int32_t p, C* F) {
+    const ChunkMeta m = a.meta[p / CHUNK];
+    const int l = p & (CHUNK - 1);
+    if ((m.irr_mask >> l) & 1u) {
+      pull_irregular(a, p, m.irr_base + rank_below(m.irr_mask, l), F);
+    } else {
+      const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
+      pull_regular(a, p, segment_of(m, l), bm, F);
+    }
+  }
+
+  __device__ static void load_direct(const ColArgs& a, int32_t p, C* F) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double f = a.f_direct ? a.f_direct[int64_t(q) * a.npad + p] : L::w[q];
+      F[q] = SHIFT ? C(f - L::w[q]) : C(f);
+    }
+  }
+
+  __device__ static void write_direct(const ColArgs& a, int32_t p, const C* F) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a.f_out[int64_t(q) * a.npad + p] = SHIFT ? double(F[q]) + L::w[q] : double(F[q]);
+  }
+
+  template <bool FORCE, bool NT>
+  __device__ static void collide_store(const ColArgs& a, int32_t p, C* F) {
+    C drho, ux, uy, uz;
+    const C frc[3] = {C(a.force[0]), C(a.force[1]), C(a.force[2])};
+    const C alpha = (FORCE && a.porosity) ? C(a.porosity[p]) : C(0);
+    collide_ext<L, C, SHIFT, FORCE>(F, C(a.omega), frc, alpha, false, drho, ux, uy, uz);
+    // NT: nontemporal stores (a compile-time choice: a runtime branch
+    // between two otherwise identical stores is merged and loses the hint).
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      if constexpr (NT) P::st_nt(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+      else P::st(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+    }
+  }
+
+  // Wall shear stress of near-wall node i from its pulled,
+  // boundary-treated populations F:
+  //   tau_i = C sum_a (f_a - feq_a) (c_a.n) (c_ai - (c_a.n) n_i)
+  // (Simulation.cpp:775-797 with its inner sums written out). feq is the
+  // equilibrium of the collision that produced this step's input; density
+  // and velocity are recovered from the node's own post-collision
+  // populations, which BGK and the Guo source leave with the same density
+  // and with momentum rho u (1 - alpha/2) + F/2.
+  struct Eq {
+    C s, rho, ux, uy, uz;
+  };
+
+  // Density and velocity of the collision that produced slot p's input
+  // (read before the pull so that only five values stay live across it).
+  template <bool FORCE>
+  __device__ static Eq wss_equilibrium(const ColArgs& a, int32_t p) {
+    C s = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const C v = P::ld(a.fs_in, int64_t(q) * a.npad + p);
+      s += v;
+      if (L::cx[q] > 0) jx += v;
+      if (L::cx[q] < 0) jx -= v;
+      if (L::cy[q] > 0) jy += v;
+      if (L::cy[q] < 0) jy -= v;
+      if (L::cz[q] > 0) jz += v;
+      if (L::cz[q] < 0) jz -= v;
+    }
+    const C rho = SHIFT ? C(1) + s : s;
+    C den = rho;
+    if (FORCE) {
+      const C alpha = a.porosity ? C(a.porosity[p]) : C(0);
+      jx -= C(0.5) * C(a.force[0]);
+      jy -= C(0.5) * C(a.force[1]);
+      jz -= C(0.5) * C(a.force[2]);
+      den = rho * (C(1) - C(0.5) * alpha);
+    }
+    return Eq{s, rho, jx / den, jy / den, jz / den};
+  }
+
+  __device__ static void wss_node(const ColArgs& a, int32_t i, const Eq& e, const C* F) {
+    const C ux = e.ux, uy = e.uy, uz = e.uz, rho = e.rho, s = e.s;
+    const C usq = ux * ux + uy * uy + uz * uz;
+    const int64_t m = a.n_nw;
+    const double n[3] = {a.nw_normal[i], a.nw_normal[m + i], a.nw_normal[2 * m + i]};
+    double tau[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int al = 0; al < Q; ++al) {
+      const double fne = double(F[al] - geq_of<L, C, SHIFT>(al, SHIFT ? s : C(0), rho, ux, uy, uz, usq));
+      const double cn = L::cx[al] * n[0] + L::cy[al] * n[1] + L::cz[al] * n[2];
+      tau[0] += fne * cn * (L::cx[al] - cn * n[0]);
+      tau[1] += fne * cn * (L::cy[al] - cn * n[1]);
+      tau[2] += fne * cn * (L::cz[al] - cn * n[2]);
+    }
+    const double t0 = a.wss_C * tau[0], t1 = a.wss_C * tau[1], t2 = a.wss_C * tau[2];
+    const double W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    a.nw_wss[i] = W;
+    a.nw_wss[m + i] += t0;
+    a.nw_wss[2 * m + i] += t1;
+    a.nw_wss[3 * m + i] += t2;
+    a.nw_wss[4 * m + i] += W;
+    if (a.obj_acc) {  // objective: weighted sum of |tau|^2 over the chosen wall segment
+      const double wt = a.obj_w[i];
+      if (wt != 0.0) a.obj_acc[i] += wt * W * W;
+    }
+  }
+
+  __device__ static bool is_boundary_kind(const ColArgs& a, uint32_t kind) {
+    return (kind == 2 && a.inlet_mode != 2) || (kind == 3 && a.outlet_mode != 2);
+  }
+
+  // Slot p = lane l of a chunk whose metadata m is wave-uniform.
+  template <int MODE, bool FORCE, bool NT>
+  __device__ static void main_slot(const ColArgs& a, int32_t p, int l, const ChunkMeta& m) {
+    C F[Q];
+    const bool irr = (m.irr_mask >> l) & 1u;
+    const int32_t row = irr ? m.irr_base + rank_below(m.irr_mask, l) : -1;
+    if (MODE != KMODE_INIT && irr && is_boundary_kind(a, a.irr_kind[row] & ROW_KIND_MASK)) return;
+    const bool nw = MODE == KMODE_STEP && a.wss_on && ((m.nw_mask >> l) & 1u);
+    Eq e{};
+    if (nw) e = wss_equilibrium<FORCE>(a, p);
+    if (MODE == KMODE_INIT) {
+      load_direct(a, p, F);
+    } else if (irr) {
+      pull_irregular(a, p, row, F);
+    } else {
+      const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
+      pull_regular(a, p, segment_of(m, l), bm, F);
+    }
+    if (nw) wss_node(a, m.nw_base + rank_below(m.nw_mask, l), e, F);
+    if (MODE == KMODE_MATERIALIZE) {
+      write_direct(a, p, F);
+      return;
+    }
+    collide_store<FORCE, NT>(a, p, F);
+  }
+
+  __device__ static C inlet_speed(const ColArgs& a, int32_t i) {
+    double u = 0.0;
+    for (int m = 0; m < a.n_modes; ++m) u += a.coef[m] * a.inlet_basis[int64_t(m) * a.n_inlet + i];
+    return C(u);
+  }
+
+  // Non-equilibrium extrapolation in place on the interior neighbour's
+  // post-stream populations G: G <- feq(rho_b, u_b) + (G - feq(rho_G, u_G)).
+  // rho_b <= 0 takes rho_G; use_G_velocity takes u_b = u_G.
+  __device__ static void nee_inplace(C* G, C rho_b, C bx, C by, C bz, bool use_G_velocity) {
+#pragma clang fp contract(fast)
+    C s, jx, jy, jz;
+    moments<L, C>(G, s, jx, jy, jz);
+    const C rg = SHIFT ? C(1) + s : s;
+    const C gx = jx / rg, gy = jy / rg, gz = jz / rg;
+    if (use_G_velocity) {
+      bx = gx;
+      by = gy;
+      bz = gz;
+    }
+    if (rho_b <= C(0)) rho_b = rg;
+    const C usq_b = bx * bx + by * by + bz * bz, usq_g = gx * gx + gy * gy + gz * gz;
+    const C db = SHIFT ? rho_b - C(1) : C(0), dg = SHIFT ? s : C(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      G[q] = geq_of<L, C, SHIFT>(q, db, rho_b, bx, by, bz, usq_b) +
+             (G[q] - geq_of<L, C, SHIFT>(q, dg, rg, gx, gy, gz, usq_g));
+  }
+
+  // Inlet/outlet node p (always an explicit row).
+  template <int MODE, bool FORCE, bool NT>
+  __device__ static void boundary_slot(const ColArgs& a, int32_t p, int32_t bi) {
+    const ChunkMeta m = a.meta[p / CHUNK];
+    const int l = p & (CHUNK - 1);
+    if (!((m.irr_mask >> l) & 1u)) return;
+    const int32_t row = m.irr_base + rank_below(m.irr_mask, l);
+    const uint32_t kind = a.irr_kind[row] & ROW_KIND_MASK;
+    const int32_t u = a.irr_bcn[row];
+    const bool nw = MODE == KMODE_STEP && a.wss_on && ((m.nw_mask >> l) & 1u);
+    Eq e{};
+    if (nw) e = wss_equilibrium<FORCE>(a, p);
+    C F[Q];
+    if (kind == 2) {
+      if (a.inlet_mode == INLET_NEE && u >= 0) {
+        pull_any(a, u, F);
+        const C U = inlet_speed(a, a.irr_inlet[row]);
+        nee_inplace(F, C(-1), -C(a.inlet_normal[0]) * U, -C(a.inlet_normal[1]) * U, -C(a.inlet_normal[2]) * U,
+                    false);
+      } else {
+        pull_irregular(a, p, row, F);
+        if (a.inlet_mode == INLET_ZOUHE) {
+          if constexpr (Q == 19) zouhe_ref<C, SHIFT>(F, inlet_speed(a, a.irr_inlet[row]), a.inlet_normal);
+        }
+      }
+    } else if (u >= 0) {  // outlet with a node behind it
+      pull_any(a, u, F);
+      if (a.outlet_mode == OUTLET_REF) {
+        // Simulation::ApplyOutletBc: every population of the node behind,
+        // after that node's own inlet rule (Zou-He runs first).
+        const ChunkMeta mu = a.meta[u / CHUNK];
+        const int lu = u & (CHUNK - 1);
+        if (((mu.irr_mask >> lu) & 1u) && a.inlet_mode == INLET_ZOUHE) {
+          const int32_t urow = mu.irr_base + rank_below(mu.irr_mask, lu);
+          if ((a.irr_kind[urow] & ROW_KIND_MASK) == 2) {
+            if constexpr (Q == 19) zouhe_ref<C, SHIFT>(F, inlet_speed(a, a.irr_inlet[urow]), a.inlet_normal);
+          }
+        }
+      } else {
+        nee_inplace(F, C(a.outlet_rho), C(0), C(0), C(0), true);
+      }
+    } else {
+      pull_irregular(a, p, row, F);
+    }
+    // The stress follows every boundary rule, as in the reference order.
+    if (nw) wss_node(a, m.nw_base + rank_below(m.nw_mask, l), e, F);
+    if (MODE == KMODE_STEP && a.obj_rho) {  // objective: density sums for the pressure drop
+      C s = C(0);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) s += F[q];
+      a.obj_rho[bi] += double(SHIFT ? C(1) + s : s);
+    }
+    if (MODE == KMODE_MATERIALIZE) {
+      write_direct(a, p, F);
+      return;
+    }
+    collide_store<FORCE, NT>(a, p, F);
+  }
+};
+
+}  // namespace colk
+}  // namespace lbm

@@ -107,6 +107,10 @@ struct ColArgs {
   const float* porosity;  // per slot, or null
   const double* f_direct; // init input (FP64 unshifted, q * npad + slot), null = w_q
   double* f_out;          // materialize output
+  // Objective accumulation (adjoint forward pass), all null otherwise.
+  double* obj_acc;        // n_nw: sum over steps of obj_w |tau|^2
+  const double* obj_w;    // n_nw: wall-segment weight (0 = outside)
+  double* obj_rho;        // n_bslots: sum over steps of the post-rule density
 };
 
 hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st);

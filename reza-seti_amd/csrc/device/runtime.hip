@@ -14,7 +14,9 @@
 #include <vector>
 
 #include "../capi_internal.h"
+#include "../host/checkpoint.h"
 #include "../lattice.h"
+#include "adjoint.h"
 #include "columns.h"
 #include "kernels.h"
 
@@ -119,7 +121,21 @@ struct lbm_solver {
   double* nw_wss = nullptr;
   int64_t n_irr = 0, n_nw = 0;
   std::vector<int32_t> nw_rows;      // near-wall list order -> near-wall slot rank
+  std::vector<int32_t> bsl_slot, bsl_kind, bsl_up, bsl_inlet;  // inlet/outlet list (bslots order)
   ColArgs cbase_args{};
+
+  // Adjoint forward record (lbm_adjoint_forward).
+  struct Adjoint {
+    bool recorded = false;
+    int32_t T = 0, K = 0, nck = 0, n_in = 0, n_out = 0;
+    std::vector<void*> ckpt;          // h_0, h_K, h_2K, ..., h_T
+    std::vector<double> coef;         // T * n_modes
+    std::vector<double> g0;           // initial f (node-major), empty = w_q
+    double wss_weight = 0.0, dp_weight = 0.0;
+    double* obj_w = nullptr;          // n_nw
+    double* obj_acc = nullptr;        // n_nw
+    double* obj_rho = nullptr;        // n_bslots
+  } adj;
 };
 
 namespace {
@@ -141,7 +157,17 @@ int dalloc_raw(lbm_solver* s, void** p, size_t bytes) {
   return LBM_OK;
 }
 
+void adjoint_release(lbm_solver* s) {
+  for (void* p : s->adj.ckpt)
+    if (p) (void)hipFree(p);
+  void* ptrs[] = {s->adj.obj_w, s->adj.obj_acc, s->adj.obj_rho};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  s->adj = lbm_solver::Adjoint{};
+}
+
 void free_all(lbm_solver* s) {
+  adjoint_release(s);
   void* ptrs[] = {s->fs[0], s->fs[1], s->fprev[0], s->fprev[1], s->src, s->sflags, s->bc_nbr, s->inlet_slot,
                   s->inlet_basis, s->normal, s->rho_u, s->wss, s->porosity, s->err, s->f_tmp, s->partials,
                   s->red_out, s->meta, s->seg_delta, s->bbm, s->irr_src, s->irr_kind, s->irr_bcn,
@@ -503,6 +529,15 @@ int create_columns(lbm_solver* s, const Tables& t) {
   c.wss_C = wss_C;
   c.bslots = s->bslots;
   c.n_bslots = static_cast<int32_t>(bsl.size());
+  s->bsl_slot = bsl;
+  s->bsl_kind.resize(bsl.size());
+  s->bsl_up.resize(bsl.size());
+  s->bsl_inlet.resize(bsl.size());
+  for (size_t i = 0; i < bsl.size(); ++i) {
+    s->bsl_kind[i] = t.kind[bsl[i]];
+    s->bsl_up[i] = t.bc_nbr[bsl[i]];
+    s->bsl_inlet[i] = inlet_index[bsl[i]];
+  }
   c.n_inlet = static_cast<int32_t>(ninl);
   c.n_modes = 1;
   c.inlet_basis = s->inlet_basis;
@@ -526,6 +561,23 @@ int create_columns(lbm_solver* s, const Tables& t) {
   a.err_flags = s->err;
   return LBM_OK;
 }
+
+// Device buffers of one reverse pass, released on every exit path.
+struct AdjointScratch {
+  std::vector<void*> bufs;
+  ~AdjointScratch() {
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+  }
+  template <class T>
+  hipError_t alloc(T** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+    if (e == hipSuccess) bufs.push_back(*p);
+    return e;
+  }
+};
 
 }  // namespace
 
@@ -758,11 +810,19 @@ int lbm_set_inlet_profile(lbm_solver* s, int32_t n_modes, const double* basis) {
 int lbm_set_porosity(lbm_solver* s, const float* alpha) {
   if (!s) return fail(LBM_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->prm.device));
+  // Before the first step the initial collision is redone with the new
+  // field, as if it had been given at creation.
+  auto recollide = [&]() -> int {
+    if (s->stepped) return LBM_OK;
+    int rc = init_state(s, s->last_set.empty() ? nullptr : s->last_set.data());
+    if (rc == LBM_OK && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(LBM_ERR_HIP, "init failed");
+    return rc;
+  };
   if (!alpha) {
     if (s->porosity) (void)hipFree(s->porosity);
     s->porosity = nullptr;
     s->base.porosity = nullptr;
-    return LBM_OK;
+    return recollide();
   }
   if (!s->porosity) {
     int rc = dalloc(s, &s->porosity, s->npad);
@@ -774,7 +834,7 @@ int lbm_set_porosity(lbm_solver* s, const float* alpha) {
   HIP_TRY(hipMemcpyAsync(s->porosity, h.data(), h.size() * 4, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->base.porosity = s->porosity;
-  return LBM_OK;
+  return recollide();
 }
 
 int lbm_step(lbm_solver* s, int32_t nsteps) {
@@ -986,6 +1046,260 @@ int lbm_reduce(lbm_solver* s, lbm_stats* out) {
   out->wss_max = h[5];
   out->iteration = s->iter;
   out->time = s->total_runtime;
+  return LBM_OK;
+}
+
+int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const lbm_objective* obj,
+                        int32_t checkpoint_every, double* J) {
+  if (!s || !obj || !J) return fail(LBM_ERR_ARG, "null argument");
+  if (s->family != 1)
+    return fail(LBM_ERR_UNSUPPORTED, "the adjoint covers the half-way bounce-back wall rule (LBM_WALL_HALFWAY_BB)");
+  if (!(s->prm.inlet_mode == LBM_INLET_NEE_VELOCITY || s->prm.inlet_mode == LBM_INLET_NONE) ||
+      !(s->prm.outlet_mode == LBM_OUTLET_NEE_PRESSURE || s->prm.outlet_mode == LBM_OUTLET_NONE))
+    return fail(LBM_ERR_UNSUPPORTED, "the adjoint covers the NEE inlet/outlet rules (or none)");
+  if (nsteps < 1 || checkpoint_every < 1) return fail(LBM_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
+  if (s->stepped)
+    return fail(LBM_ERR_STATE, "the adjoint forward pass starts from a state set by lbm_create or lbm_set_populations");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  adjoint_release(s);
+  auto& A = s->adj;
+  const int32_t T = nsteps, K = checkpoint_every, M = s->n_modes;
+  const size_t state_bytes = static_cast<size_t>(s->Q) * s->npad * s->esize;
+  const size_t nnw = static_cast<size_t>(std::max<int64_t>(s->n_nw, 1));
+  const size_t nb = std::max<size_t>(s->bsl_slot.size(), 1);
+
+  std::vector<double> w(nnw, 0.0);
+  for (size_t i = 0; i < s->nw_rows.size(); ++i)
+    w[s->nw_rows[i]] = (obj->wss_mask && !obj->wss_mask[i]) ? 0.0 : obj->wss_weight;
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_w), nnw * 8));
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_acc), nnw * 8));
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_rho), nb * 8));
+  HIP_TRY(hipMemcpyAsync(A.obj_w, w.data(), nnw * 8, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemsetAsync(A.obj_acc, 0, nnw * 8, s->stream));
+  HIP_TRY(hipMemsetAsync(A.obj_rho, 0, nb * 8, s->stream));
+  for (int32_t k : s->bsl_kind) {
+    if (k == KIND_INLET) A.n_in++;
+    if (k == KIND_OUTLET) A.n_out++;
+  }
+  A.wss_weight = obj->wss_weight;
+  A.dp_weight = obj->dp_weight;
+  A.g0 = s->last_set;
+  A.T = T;
+  A.K = K;
+  A.nck = (T + K - 1) / K + 1;
+  A.ckpt.assign(A.nck, nullptr);
+  for (auto& p : A.ckpt) {
+    if (hipMalloc(&p, state_bytes) != hipSuccess) {
+      adjoint_release(s);
+      return fail(LBM_ERR_HIP, "not enough device memory for the checkpoints");
+    }
+  }
+  A.coef.assign(static_cast<size_t>(T) * M, 0.0);
+  HIP_TRY(hipMemcpyAsync(A.ckpt[0], s->fs[s->cur], state_bytes, hipMemcpyDeviceToDevice, s->stream));
+  const bool force = has_force(s);
+  for (int32_t t = 0; t < T; ++t) {
+    ColArgs c = col_args_for(s, s->cur);
+    for (int m = 0; m < MAX_MODES; ++m) c.coef[m] = 0.0;
+    if (coef) {
+      for (int m = 0; m < M; ++m) c.coef[m] = coef[int64_t(t) * M + m];
+    } else {
+      c.coef[0] = (s->prm.flow_type == LBM_FLOW_PULSATILE) ? inlet_velocity(s->total_runtime, 0.9) : s->prm.cte_vel;
+    }
+    for (int m = 0; m < M; ++m) A.coef[int64_t(t) * M + m] = c.coef[m];
+    c.wss_on = 1;
+    c.obj_acc = A.obj_acc;
+    c.obj_w = A.obj_w;
+    c.obj_rho = A.obj_rho;
+    HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->stream));
+    std::memcpy(s->last_coef, c.coef, sizeof(s->last_coef));
+    s->cur = 1 - s->cur;
+    s->iter++;
+    if (s->prm.flow_type == LBM_FLOW_PULSATILE) s->total_runtime += s->prm.dt;
+    s->stepped = true;
+    if ((t + 1) % K == 0 || t + 1 == T)
+      HIP_TRY(hipMemcpyAsync(A.ckpt[(t + 1 + K - 1) / K], s->fs[s->cur], state_bytes, hipMemcpyDeviceToDevice,
+                             s->stream));
+  }
+  std::vector<double> acc(nnw), rho(nb);
+  HIP_TRY(hipMemcpyAsync(acc.data(), A.obj_acc, nnw * 8, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipMemcpyAsync(rho.data(), A.obj_rho, nb * 8, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  double sw = 0.0, rin = 0.0, rout = 0.0;
+  for (size_t i = 0; i < static_cast<size_t>(s->n_nw); ++i) sw += acc[i];
+  for (size_t i = 0; i < s->bsl_slot.size(); ++i) {
+    if (s->bsl_kind[i] == KIND_INLET) rin += rho[i];
+    if (s->bsl_kind[i] == KIND_OUTLET) rout += rho[i];
+  }
+  double dp = 0.0;
+  if (A.n_in) dp += rin / A.n_in;
+  if (A.n_out) dp -= rout / A.n_out;
+  *J = (sw + A.dp_weight * dp / 3.0) / T;
+  A.recorded = true;
+  return LBM_OK;
+}
+
+int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double* grad_porosity) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  auto& A = s->adj;
+  if (!A.recorded) return fail(LBM_ERR_STATE, "no adjoint forward pass recorded");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  const int Q = s->Q;
+  const int32_t T = A.T, K = A.K, M = s->n_modes;
+  const int64_t npad = s->npad;
+  const size_t state_bytes = static_cast<size_t>(Q) * npad * s->esize;
+  const size_t aes = s->prec == 0 ? 8 : 4;  // adjoint arithmetic: FP64 or FP32
+  const size_t nnw = static_cast<size_t>(std::max<int64_t>(s->n_nw, 1));
+  const size_t nb = std::max<size_t>(s->bsl_slot.size(), 1);
+  ColArgs cb = col_args_for(s, 0);
+  const int32_t n_inlet = cb.n_inlet;
+
+  AdjointScratch S;
+  void *rho0 = nullptr, *rho1 = nullptr, *mb = nullptr, *scratch = nullptr;
+  double *own = nullptr, *galpha = nullptr, *gamma = nullptr, *g0 = nullptr, *gcoef = nullptr;
+  int32_t *tgt = nullptr, *tgt_off = nullptr, *tgt_b = nullptr;
+  HIP_TRY(S.alloc(&rho0, static_cast<size_t>(Q) * npad * aes));
+  HIP_TRY(S.alloc(&rho1, static_cast<size_t>(Q) * npad * aes));
+  HIP_TRY(S.alloc(&own, nnw * Q * 8));
+  HIP_TRY(S.alloc(&mb, nb * Q * aes));
+  HIP_TRY(S.alloc(&galpha, static_cast<size_t>(npad) * 8));
+  HIP_TRY(S.alloc(&gamma, static_cast<size_t>(T) * n_inlet * 8));
+  HIP_TRY(S.alloc(&gcoef, static_cast<size_t>(T) * std::max(M, 1) * 8));
+  HIP_TRY(hipMemsetAsync(galpha, 0, static_cast<size_t>(npad) * 8, s->stream));
+  HIP_TRY(hipMemsetAsync(gamma, 0, static_cast<size_t>(T) * n_inlet * 8, s->stream));
+  HIP_TRY(hipMemsetAsync(rho0, 0, static_cast<size_t>(Q) * npad * aes, s->stream));
+  HIP_TRY(hipMemsetAsync(rho1, 0, static_cast<size_t>(Q) * npad * aes, s->stream));
+  if (!A.g0.empty()) {
+    std::vector<double> soa(static_cast<size_t>(Q) * npad, 0.0);
+    const lbm::LatticeView L = lbm::lattice_view(Q);
+    for (int64_t slot = 0; slot < npad; ++slot) {
+      const int32_t n = s->node_of[slot];
+      for (int q = 0; q < Q; ++q) soa[static_cast<size_t>(q) * npad + slot] = n >= 0 ? A.g0[int64_t(n) * Q + q] : L.w[q];
+    }
+    HIP_TRY(S.alloc(&g0, soa.size() * 8));
+    HIP_TRY(hipMemcpyAsync(g0, soa.data(), soa.size() * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+  }
+
+  // Upstream nodes of extrapolated inlet/outlet nodes, grouped (CSR).
+  std::vector<std::pair<int32_t, int32_t>> up;  // (upstream slot, bslots index)
+  for (size_t i = 0; i < s->bsl_slot.size(); ++i) {
+    const int32_t u = s->bsl_up[i], k = s->bsl_kind[i];
+    const bool nee = u >= 0 && ((k == KIND_INLET && s->prm.inlet_mode == LBM_INLET_NEE_VELOCITY) ||
+                                (k == KIND_OUTLET && s->prm.outlet_mode == LBM_OUTLET_NEE_PRESSURE));
+    if (nee) up.emplace_back(s->slot_of[u], static_cast<int32_t>(i));
+  }
+  std::sort(up.begin(), up.end());
+  std::vector<int32_t> h_tgt, h_off, h_b;
+  for (size_t i = 0; i < up.size(); ++i) {
+    if (i == 0 || up[i].first != up[i - 1].first) {
+      h_tgt.push_back(up[i].first);
+      h_off.push_back(static_cast<int32_t>(i));
+    }
+    h_b.push_back(up[i].second);
+  }
+  h_off.push_back(static_cast<int32_t>(up.size()));
+  HIP_TRY(S.alloc(&tgt, h_tgt.size() * 4));
+  HIP_TRY(S.alloc(&tgt_off, h_off.size() * 4));
+  HIP_TRY(S.alloc(&tgt_b, h_b.size() * 4));
+  if (!h_tgt.empty()) {
+    HIP_TRY(hipMemcpyAsync(tgt, h_tgt.data(), h_tgt.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(tgt_b, h_b.data(), h_b.size() * 4, hipMemcpyHostToDevice, s->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(tgt_off, h_off.data(), h_off.size() * 4, hipMemcpyHostToDevice, s->stream));
+
+  // State buffers for the binomial schedule.
+  const int32_t max_slots = std::max<int32_t>(K - 1, 0);
+  if (slots <= 0 || slots > max_slots) {
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const size_t reserve = size_t(1) << 30;
+    const int64_t fit = free_b > reserve ? static_cast<int64_t>((free_b - reserve) / state_bytes) : 0;
+    const int32_t want = slots <= 0 ? max_slots : std::min(slots, max_slots);
+    slots = static_cast<int32_t>(std::min<int64_t>(want, fit));
+  }
+  std::vector<void*> slot_buf(slots, nullptr);
+  for (auto& p : slot_buf) HIP_TRY(S.alloc(&p, state_bytes));
+  HIP_TRY(S.alloc(&scratch, state_bytes));
+
+  const bool force = has_force(s);
+  ColArgs cf = cb;  // forward recomputation: no stress, no objective
+  cf.wss_on = 0;
+  cf.obj_acc = nullptr;
+  cf.obj_w = nullptr;
+  cf.obj_rho = nullptr;
+  auto advance = [&](void* src, int32_t from, int32_t to, void* dst) -> int {
+    for (int32_t k = from; k < to; ++k) {
+      void* out = (k == to - 1) ? dst : s->fs[(k - from) & 1];
+      ColArgs c = cf;
+      c.fs_in = src;
+      c.fs_out = out;
+      for (int m = 0; m < MAX_MODES; ++m) c.coef[m] = m < M ? A.coef[int64_t(k) * M + m] : 0.0;
+      HIP_TRY(launch_columns(Q, s->prec, KMODE_STEP, force, c, s->stream));
+      src = out;
+    }
+    return LBM_OK;
+  };
+  AdjArgs a{};
+  a.c = cb;
+  a.c.obj_acc = nullptr;
+  a.c.obj_rho = nullptr;
+  a.own = own;
+  a.mb = mb;
+  a.g_alpha = galpha;
+  a.inv_T = 1.0 / T;
+  a.dp_in = A.n_in ? A.dp_weight / (3.0 * T * A.n_in) : 0.0;
+  a.dp_out = A.n_out ? A.dp_weight / (3.0 * T * A.n_out) : 0.0;
+  a.obj_w = A.obj_w;
+  a.n_tgt = static_cast<int32_t>(h_tgt.size());
+  a.tgt = tgt;
+  a.tgt_off = tgt_off;
+  a.tgt_b = tgt_b;
+  void* rho[2] = {rho0, rho1};
+  auto reverse = [&](int32_t t, void* h) -> int {
+    AdjArgs x = a;
+    x.mode = ADJ_STEP;
+    x.c.fs_in = h;
+    for (int m = 0; m < MAX_MODES; ++m) x.c.coef[m] = m < M ? A.coef[int64_t(t - 1) * M + m] : 0.0;
+    x.rho_in = t == T ? nullptr : rho[(t + 1) & 1];
+    x.rho_out = rho[t & 1];
+    x.gamma = gamma + int64_t(t - 1) * n_inlet;
+    HIP_TRY(launch_adjoint(Q, s->prec, x, s->stream));
+    HIP_TRY(launch_adjoint_boundary(Q, s->prec, x, s->stream));
+    return LBM_OK;
+  };
+  for (int32_t j = A.nck - 2; j >= 0; --j) {
+    const int32_t l = j * K, n = std::min(K, T - l);
+    auto buffer = [&](int32_t id) -> void* {
+      return id == REV_BASE ? A.ckpt[j] : (id == REV_SCRATCH ? scratch : slot_buf[id]);
+    };
+    for (const RevAction& act : revolve_schedule(l, n, slots)) {
+      int rc = act.op == REV_ADVANCE ? advance(buffer(act.buf_from), act.from, act.to, buffer(act.buf_to))
+                                     : reverse(act.to, buffer(act.buf_from));
+      if (rc) return rc;
+    }
+  }
+  {
+    AdjArgs x = a;  // lambda_0 and the initial collision's porosity term
+    x.mode = ADJ_FINAL;
+    x.c.fs_in = A.ckpt[0];
+    x.rho_in = rho[1];
+    x.rho_out = nullptr;
+    x.g0 = g0;
+    HIP_TRY(launch_adjoint(Q, s->prec, x, s->stream));
+  }
+  if (grad_coef) {
+    HIP_TRY(launch_coef_gradient(gamma, cb.inlet_basis, T, n_inlet, M, gcoef, s->stream));
+    HIP_TRY(hipMemcpyAsync(grad_coef, gcoef, static_cast<size_t>(T) * M * 8, hipMemcpyDeviceToHost, s->stream));
+  }
+  std::vector<double> ga;
+  if (grad_porosity) {
+    ga.resize(npad);
+    HIP_TRY(hipMemcpyAsync(ga.data(), galpha, static_cast<size_t>(npad) * 8, hipMemcpyDeviceToHost, s->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(s->fs[s->cur], A.ckpt[A.nck - 1], state_bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (grad_porosity)
+    for (int64_t n = 0; n < s->N; ++n) grad_porosity[n] = ga[s->slot_of[n]];
   return LBM_OK;
 }
 

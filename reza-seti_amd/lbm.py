@@ -79,6 +79,10 @@ class SolverInfo(C.Structure):
                 ("precision", C.c_int32), ("bytes_per_update", C.c_double)]
 
 
+class Objective(C.Structure):
+    _fields_ = [("wss_weight", C.c_double), ("dp_weight", C.c_double), ("wss_mask", C.c_void_p)]
+
+
 _lib = None
 
 # (name, argtypes, restype) of every exported symbol; also the list the
@@ -119,6 +123,9 @@ SIGNATURES = [
     ("lbm_destroy", [_P], None),
     ("lbm_get_solver_info", [_P, _P], C.c_int),
     ("lbm_copy_probe", [C.c_int32, C.c_int64, C.c_int32, _P], C.c_int),
+    ("lbm_adjoint_forward", [_P, C.c_int32, _P, _P, C.c_int32, _P], C.c_int),
+    ("lbm_adjoint_backward", [_P, C.c_int32, _P, _P], C.c_int),
+    ("lbm_checkpoint_schedule", [C.c_int32, C.c_int32, _P, C.c_int64, _P], C.c_int),
     ("lbm_last_error", [], C.c_char_p),
 ]
 
@@ -128,6 +135,16 @@ def copy_probe(nbytes, iters=20, device=0):
     g = C.c_double()
     _check(lib().lbm_copy_probe(device, int(nbytes), int(iters), C.byref(g)))
     return g.value
+
+
+def checkpoint_schedule(nsteps, slots):
+    """Binomial reversal schedule of one checkpoint interval: rows of
+    (op 0 advance / 1 reverse, from, to, buffer from, buffer to)."""
+    n = C.c_int64()
+    _check(lib().lbm_checkpoint_schedule(int(nsteps), int(slots), None, 0, C.byref(n)))
+    out = np.zeros((max(n.value, 1), 5), np.int32)
+    _check(lib().lbm_checkpoint_schedule(int(nsteps), int(slots), _ptr(out), n.value, C.byref(n)))
+    return out[:n.value]
 
 
 def lib():
@@ -314,6 +331,7 @@ class Solver:
         self._h = h
         self.q = geom.q
         self.n = geom.n
+        self.n_modes = 1
 
     def step(self, nsteps=1):
         _check(lib().lbm_step(self._h, int(nsteps)))
@@ -336,6 +354,7 @@ class Solver:
     def set_inlet_profile(self, basis):
         basis = np.ascontiguousarray(np.atleast_2d(basis), dtype=np.float64)
         _check(lib().lbm_set_inlet_profile(self._h, basis.shape[0], _ptr(basis)))
+        self.n_modes = basis.shape[0]
 
     def set_porosity(self, alpha):
         if alpha is None:
@@ -384,6 +403,31 @@ class Solver:
         s = Stats()
         _check(lib().lbm_reduce(self._h, C.byref(s)))
         return s
+
+    def adjoint_forward(self, nsteps, coef=None, wss_weight=1.0, dp_weight=0.0, wss_mask=None,
+                        checkpoint_every=200):
+        """Forward pass with checkpoints; returns the objective J."""
+        obj = Objective(float(wss_weight), float(dp_weight), None)
+        if wss_mask is not None:
+            self._mask = np.ascontiguousarray(wss_mask, dtype=np.uint8)
+            obj.wss_mask = self._mask.ctypes.data
+        cp = None
+        if coef is not None:
+            self._coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(nsteps, self.n_modes)
+            cp = _ptr(self._coef)
+        J = C.c_double()
+        _check(lib().lbm_adjoint_forward(self._h, int(nsteps), cp, C.byref(obj), int(checkpoint_every),
+                                         C.byref(J)))
+        self._adj_steps = int(nsteps)
+        self._adj_modes = self.n_modes
+        return J.value
+
+    def adjoint_backward(self, slots=0):
+        """Reverse pass: (dJ/dcoef [steps, modes], dJ/dalpha [nodes])."""
+        gc = np.zeros((self._adj_steps, self._adj_modes))
+        ga = np.zeros(self.n)
+        _check(lib().lbm_adjoint_backward(self._h, int(slots), _ptr(gc), _ptr(ga)))
+        return gc, ga
 
     def info(self):
         i = SolverInfo()

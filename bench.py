@@ -97,6 +97,107 @@ def run_precision(geom, basis, prec, steps, warmup):
     return s, wall, ev_ms
 
 
+ADJ_STEPS, ADJ_EVERY, ADJ_MODES = 4000, 200, 16
+ADJ_SEGMENT_S0, ADJ_SEGMENT_LEN = 300.0, 20.0
+
+
+def adjoint_case(geom, info):
+    """Adjoint workload on the same vessel: 16 Zernike inlet coefficients
+    (seed 3) times a pulsatile factor, Brinkman porosity 1e-3 |N(0,1)|
+    (seed 4), objective = time-averaged sum of |WSS|^2 over a 20-lu segment of
+    one daughter branch plus the inlet-outlet pressure drop."""
+    ijk, kind, _ = geom.nodes()
+    b = geom.info.bbox
+    h = geom.info.spacing
+    pts = np.array([b[0], b[2], b[4]])[None, :] + ijk * h
+    inl = np.where(kind == lbm.KIND_INLET)[0]
+    x = pts[inl, 0] - info["parent_axis"][0]
+    y = pts[inl, 1] - info["parent_axis"][1]
+    rho = np.clip(np.hypot(x, y) / info["parent_radius"], 0.0, 1.0)
+    basis = workloads.zernike_basis(rho, np.arctan2(y, x), ADJ_MODES)
+    rng = np.random.default_rng(3)
+    theta = np.concatenate([[U_MEAN], 0.1 * U_MEAN * rng.standard_normal(ADJ_MODES - 1)])
+    alpha = (1e-3 * np.abs(np.random.default_rng(4).standard_normal(geom.n))).astype(np.float32)
+    nw, _ = geom.nearwall()
+    mask = workloads.branch_segment_mask(pts[nw], info["branch_point"], info["daughter_dirs"][0], ADJ_SEGMENT_S0,
+                                         ADJ_SEGMENT_LEN, info["daughter_radius"] + 6.0).astype(np.uint8)
+    wave = 1.0 + 0.5 * np.sin(2.0 * math.pi * np.arange(ADJ_STEPS) / PERIOD)
+    return dict(basis=basis, theta=theta, alpha=alpha, mask=mask, wave=wave)
+
+
+def adjoint_solver(geom, case, prec):
+    s = lbm.Solver(geom, precision=prec, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NEE_VELOCITY,
+                   outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=tau_for(), dt=1e-5, parity_checks=0, wss_every=1)
+    s.set_inlet_profile(case["basis"])
+    s.set_porosity(case["alpha"])
+    return s
+
+
+def run_adjoint(geom, info, steps=ADJ_STEPS):
+    """Times forward, adjoint forward (checkpointed) and reverse passes in FP32
+    and checks the gradient against the FP64 adjoint and against central
+    differences of the FP64 forward along a random inlet direction."""
+    case = adjoint_case(geom, info)
+    wave = case["wave"][:steps]
+    coef = lambda th: wave[:, None] * th[None, :]  # noqa: E731
+    # Objective weights: the stress term scaled to the size of the pressure term.
+    cal = adjoint_solver(geom, case, lbm.PREC_F32)
+    jw = cal.adjoint_forward(ADJ_EVERY, coef(case["theta"])[:ADJ_EVERY], wss_weight=1.0, dp_weight=0.0,
+                             wss_mask=case["mask"], checkpoint_every=ADJ_EVERY)
+    cal.close()
+    cal = adjoint_solver(geom, case, lbm.PREC_F32)
+    jp = cal.adjoint_forward(ADJ_EVERY, coef(case["theta"])[:ADJ_EVERY], wss_weight=0.0, dp_weight=1.0,
+                             checkpoint_every=ADJ_EVERY)
+    cal.close()
+    w_wss = abs(jp / jw) if jw > 0 else 1.0
+    obj = dict(wss_weight=w_wss, dp_weight=1.0, wss_mask=case["mask"], checkpoint_every=ADJ_EVERY)
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        return r, time.perf_counter() - t0
+
+    fwd = adjoint_solver(geom, case, lbm.PREC_F32)
+    c = coef(case["theta"])
+    fwd.step_coefficients(c[:10])  # warm the kernels
+    fwd.close()
+    fwd = adjoint_solver(geom, case, lbm.PREC_F32)
+    _, t_fwd = timed(lambda: (fwd.step_coefficients(c), fwd.synchronize()))
+    fwd.close()
+    a32 = adjoint_solver(geom, case, lbm.PREC_F32)
+    J32, t_af = timed(lambda: a32.adjoint_forward(steps, c, **obj))
+    (gc32, ga32), t_bw = timed(lambda: a32.adjoint_backward(0))
+    a32.close()
+    a64 = adjoint_solver(geom, case, lbm.PREC_F64)
+    J64 = a64.adjoint_forward(steps, c, **obj)
+    gc64, ga64 = a64.adjoint_backward(0)
+    a64.close()
+    gt32 = (wave[:, None] * gc32).sum(0)
+    gt64 = (wave[:, None] * gc64).sum(0)
+    v = np.random.default_rng(7).standard_normal(ADJ_MODES)
+    v /= np.linalg.norm(v)
+    hstep = 1e-4 * np.linalg.norm(case["theta"])
+    Jpm = []
+    for sgn in (1.0, -1.0):
+        sol = adjoint_solver(geom, case, lbm.PREC_F64)
+        Jpm.append(sol.adjoint_forward(steps, coef(case["theta"] + sgn * hstep * v), **obj))
+        sol.close()
+    fd = (Jpm[0] - Jpm[1]) / (2.0 * hstep)
+    ad = float(gt64 @ v)
+    return {
+        "workload": f"adjoint on the same vessel: {steps} FP32 steps, checkpoint every {ADJ_EVERY}, "
+                    f"{ADJ_MODES} Zernike inlet coefficients + per-node Brinkman porosity, "
+                    f"J = mean sum |WSS|^2 over a {ADJ_SEGMENT_LEN:g}-lu daughter segment "
+                    f"({int(case['mask'].sum())} near-wall nodes) + pressure drop",
+        "forward_s": round(t_fwd, 4), "adjoint_forward_s": round(t_af, 4), "backward_s": round(t_bw, 4),
+        "adjoint_over_forward": round(t_bw / t_fwd, 3), "gradient_over_forward": round((t_af + t_bw) / t_fwd, 3),
+        "J_f32": J32, "J_f64": J64,
+        "fd_directional_rel": abs(ad - fd) / abs(fd) if fd != 0 else None,
+        "f32_vs_f64_inlet_grad_rel_l2": float(np.linalg.norm(gt32 - gt64) / np.linalg.norm(gt64)),
+        "f32_vs_f64_porosity_grad_rel_l2": float(np.linalg.norm(ga32 - ga64) / np.linalg.norm(ga64)),
+    }
+
+
 def cpu_baseline(geom, budget_s=12.0):
     """The oracle (FP64 restatement of the reference step, OpenMP) on the host
     cores, on a bounded sample of the same workload."""
@@ -147,6 +248,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="f32,f64,f16")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=ADJ_STEPS)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole lattice on a single MI355X "
@@ -203,6 +306,12 @@ def main():
     if "f16" in runs:
         out["fp16_mlups"] = round(runs["f16"]["mlups"], 1)
     out["adjoint_over_forward"] = None
+    if not args.no_adjoint:
+        adj = run_adjoint(geom, info, args.adjoint_steps)
+        print(f"# adjoint: forward {adj['forward_s']:.3f}s, checkpointed forward {adj['adjoint_forward_s']:.3f}s, "
+              f"reverse {adj['backward_s']:.3f}s", file=sys.stderr, flush=True)
+        out["adjoint_over_forward"] = adj["adjoint_over_forward"]
+        out["adjoint"] = adj
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(geom)
     print(json.dumps(out))

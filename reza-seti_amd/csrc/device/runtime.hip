@@ -129,6 +129,7 @@ struct lbm_solver {
     bool recorded = false;
     int32_t T = 0, K = 0, nck = 0, n_in = 0, n_out = 0;
     std::vector<void*> ckpt;          // h_0, h_K, h_2K, ..., h_T
+    void* prev = nullptr;             // h_{T-1} (the input of the last step)
     std::vector<double> coef;         // T * n_modes
     std::vector<double> g0;           // initial f (node-major), empty = w_q
     double wss_weight = 0.0, dp_weight = 0.0;
@@ -160,7 +161,7 @@ int dalloc_raw(lbm_solver* s, void** p, size_t bytes) {
 void adjoint_release(lbm_solver* s) {
   for (void* p : s->adj.ckpt)
     if (p) (void)hipFree(p);
-  void* ptrs[] = {s->adj.obj_w, s->adj.obj_acc, s->adj.obj_rho};
+  void* ptrs[] = {s->adj.prev, s->adj.obj_w, s->adj.obj_acc, s->adj.obj_rho};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   s->adj = lbm_solver::Adjoint{};
@@ -1088,6 +1089,10 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
   A.K = K;
   A.nck = (T + K - 1) / K + 1;
   A.ckpt.assign(A.nck, nullptr);
+  if (hipMalloc(&A.prev, state_bytes) != hipSuccess) {
+    adjoint_release(s);
+    return fail(LBM_ERR_HIP, "not enough device memory for the checkpoints");
+  }
   for (auto& p : A.ckpt) {
     if (hipMalloc(&p, state_bytes) != hipSuccess) {
       adjoint_release(s);
@@ -1120,6 +1125,7 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
       HIP_TRY(hipMemcpyAsync(A.ckpt[(t + 1 + K - 1) / K], s->fs[s->cur], state_bytes, hipMemcpyDeviceToDevice,
                              s->stream));
   }
+  HIP_TRY(hipMemcpyAsync(A.prev, s->fs[1 - s->cur], state_bytes, hipMemcpyDeviceToDevice, s->stream));
   std::vector<double> acc(nnw), rho(nb);
   HIP_TRY(hipMemcpyAsync(acc.data(), A.obj_acc, nnw * 8, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipMemcpyAsync(rho.data(), A.obj_rho, nb * 8, hipMemcpyDeviceToHost, s->stream));
@@ -1296,7 +1302,10 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
     ga.resize(npad);
     HIP_TRY(hipMemcpyAsync(ga.data(), galpha, static_cast<size_t>(npad) * 8, hipMemcpyDeviceToHost, s->stream));
   }
+  // Restore the forward pass's last two states (the current state and the
+  // input of the last step, which lbm_get_populations re-streams).
   HIP_TRY(hipMemcpyAsync(s->fs[s->cur], A.ckpt[A.nck - 1], state_bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->fs[1 - s->cur], A.prev, state_bytes, hipMemcpyDeviceToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (grad_porosity)
     for (int64_t n = 0; n < s->N; ++n) grad_porosity[n] = ga[s->slot_of[n]];

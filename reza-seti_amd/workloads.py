@@ -98,7 +98,7 @@ def write_config(path, stl_file, *, tau, cte_vel=0.05, max_iter=2000, dt=1e-5, g
 
 
 def c0_tube(workdir, radius=10.0, length=80.0, tau=0.8, cte_vel=0.05, max_iter=2000, segments=64):
-    """Straight tube (BASELINE config 0): 64-segment cylinder, radius 10,
+    """Straight tube: 64-segment cylinder, radius 10,
     length 80, inlet cap at z = 0 (normal -z, reference Zou-He), outlet cap at
     z = length (normal +z, reference copy rule). Returns the config path."""
     os.makedirs(workdir, exist_ok=True)
@@ -322,11 +322,13 @@ def bifurcation_function(parent_radius=24.0, daughter_radius=18.0, half_angle_de
     ext = daughter_length * np.sin(a) + daughter_radius + amplitude + 2
     lo = (-ext, -(parent_radius + amplitude + 2), 0.0)
     hi = (ext, parent_radius + amplitude + 2, z_top)
-    return F, lo, hi, dict(z_top=z_top, parent_axis=(0.0, 0.0), parent_radius=parent_radius)
+    return F, lo, hi, dict(z_top=z_top, parent_axis=(0.0, 0.0), parent_radius=parent_radius,
+                           branch_point=tuple(P1), daughter_dirs=[tuple((Pd - P1) / daughter_length) for Pd in D],
+                           daughter_radius=daughter_radius)
 
 
 def c1_bifurcation(workdir, mesh_h=3.5, **kw):
-    """Y-bifurcation of BASELINE config 1 (parent r 24, daughters r 18 at 35
+    """Precision-study Y-bifurcation (parent r 24, daughters r 18 at 35
     degrees, 3-mode wall perturbation of amplitude 2, seed 1). Inlet: the z = 0
     cap (normal -z); outlet: the top cut plane (normal +z), set on the last
     grid layer inside the cap. Returns (config path, info dict)."""
@@ -344,3 +346,35 @@ def c1_bifurcation(workdir, mesh_h=3.5, **kw):
                  outlets=[dict(plane="z", coordinate=z_out, normal=(0, 0, 1))])
     info.update(n_triangles=len(tris), z_in=zmin, z_out=z_out)
     return cfg, info
+
+
+def zernike_basis(rho, phi, n_modes=16):
+    """Zernike polynomials Z_n^m in OSA/ANSI order (j = 0, 1, ...) on the unit
+    disk: radial part R_n^|m|(rho) times cos(m phi) for m >= 0, sin(|m| phi)
+    for m < 0. Returns (n_modes, len(rho))."""
+    out = []
+    n = 0
+    while len(out) < n_modes:
+        for m in range(-n, n + 1, 2):
+            if len(out) == n_modes:
+                break
+            am = abs(m)
+            R = np.zeros_like(rho)
+            for k in range((n - am) // 2 + 1):
+                c = (-1) ** k * math.factorial(n - k) / (
+                    math.factorial(k) * math.factorial((n + am) // 2 - k) * math.factorial((n - am) // 2 - k))
+                R = R + c * rho ** (n - 2 * k)
+            out.append(R * (np.cos(m * phi) if m >= 0 else np.sin(am * phi)))
+        n += 1
+    return np.stack(out)
+
+
+def branch_segment_mask(points, origin, direction, s0, length, radius):
+    """Points within `radius` of the axis origin + s * direction with
+    s0 <= s < s0 + length (a wall segment of one branch)."""
+    d = np.asarray(direction, float)
+    d = d / np.linalg.norm(d)
+    rel = np.asarray(points, float) - np.asarray(origin, float)
+    s = rel @ d
+    dist = np.linalg.norm(rel - s[:, None] * d[None, :], axis=1)
+    return (s >= s0) & (s < s0 + length) & (dist <= radius)

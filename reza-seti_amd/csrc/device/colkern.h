@@ -26,6 +26,7 @@ struct ColKern {
 
   // Pull of a regular slot p in segment `seg`; bit q of `bm` marks a missing
   // link (half-way bounce-back: the node's own population opp(q)).
+  template <bool NTL = false>
   __device__ static void pull_regular(const ColArgs& a, int32_t p, int32_t seg, uint32_t bm, C* F) {
     const int4* d4 = reinterpret_cast<const int4*>(a.seg_delta) + 2 * int64_t(seg);
     const int4 d0 = d4[0], d1 = d4[1];
@@ -35,18 +36,24 @@ struct ColKern {
       const int o = col_of(L::cx[q], L::cy[q]);
       const int32_t s = (o < 0 ? p : p + d[o]) - L::cz[q];
       const bool bb = (bm >> q) & 1u;
-      F[q] = P::ld(a.fs_in, int64_t(bb ? L::opp[q] : q) * a.npad + (bb ? p : s));
+      const int64_t idx = int64_t(bb ? L::opp[q] : q) * a.npad + (bb ? p : s);
+      if constexpr (NTL) F[q] = P::ld_nt(a.fs_in, idx);
+      else F[q] = P::ld(a.fs_in, idx);
     }
   }
 
   // Pull of an explicit row.
+  // (The same cache hint as the regular pull: the compiler merges the two
+  // branches' loads and would otherwise drop the hint.)
+  template <bool NTL = false>
   __device__ static void pull_irregular(const ColArgs& a, int32_t p, int32_t row, C* F) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int32_t s = a.irr_src[int64_t(q) * a.n_irr + row];
       const int32_t slot = s >= 0 ? s : p;
       const int qq = s >= 0 ? q : L::opp[q];
-      F[q] = P::ld(a.fs_in, int64_t(qq) * a.npad + slot);
+      if constexpr (NTL) F[q] = P::ld_nt(a.fs_in, int64_t(qq) * a.npad + slot);
+      else F[q] = P::ld(a.fs_in, int64_t(qq) * a.npad + slot);
     }
   }
 
@@ -134,7 +141,10 @@ struct ColKern {
     const C ux = e.ux, uy = e.uy, uz = e.uz, rho = e.rho, s = e.s;
     const C usq = ux * ux + uy * uy + uz * uz;
     const int64_t m = a.n_nw;
-    const double n[3] = {a.nw_normal[i], a.nw_normal[m + i], a.nw_normal[2 * m + i]};
+    // The stress arrays are touched once per step: stream them past the
+    // caches, which are left to the populations.
+    const double n[3] = {__builtin_nontemporal_load(a.nw_normal + i), __builtin_nontemporal_load(a.nw_normal + m + i),
+                         __builtin_nontemporal_load(a.nw_normal + 2 * m + i)};
     double tau[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int al = 0; al < Q; ++al) {
@@ -146,11 +156,12 @@ struct ColKern {
     }
     const double t0 = a.wss_C * tau[0], t1 = a.wss_C * tau[1], t2 = a.wss_C * tau[2];
     const double W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
-    a.nw_wss[i] = W;
-    a.nw_wss[m + i] += t0;
-    a.nw_wss[2 * m + i] += t1;
-    a.nw_wss[3 * m + i] += t2;
-    a.nw_wss[4 * m + i] += W;
+    double* acc = a.nw_wss;
+    __builtin_nontemporal_store(W, acc + i);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(acc + m + i) + t0, acc + m + i);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(acc + 2 * m + i) + t1, acc + 2 * m + i);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(acc + 3 * m + i) + t2, acc + 3 * m + i);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(acc + 4 * m + i) + W, acc + 4 * m + i);
     if (a.obj_acc) {  // objective: weighted sum of |tau|^2 over the chosen wall segment
       const double wt = a.obj_w[i];
       if (wt != 0.0) a.obj_acc[i] += wt * W * W;
@@ -162,7 +173,7 @@ struct ColKern {
   }
 
   // Slot p = lane l of a chunk whose metadata m is wave-uniform.
-  template <int MODE, bool FORCE, bool NT>
+  template <int MODE, bool FORCE, bool NT, bool NTL>
   __device__ static void main_slot(const ColArgs& a, int32_t p, int l, const ChunkMeta& m) {
     C F[Q];
     const bool irr = (m.irr_mask >> l) & 1u;
@@ -174,10 +185,10 @@ struct ColKern {
     if (MODE == KMODE_INIT) {
       load_direct(a, p, F);
     } else if (irr) {
-      pull_irregular(a, p, row, F);
+      pull_irregular<NTL>(a, p, row, F);
     } else {
       const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
-      pull_regular(a, p, segment_of(m, l), bm, F);
+      pull_regular<NTL>(a, p, segment_of(m, l), bm, F);
     }
     if (nw) wss_node(a, m.nw_base + rank_below(m.nw_mask, l), e, F);
     if (MODE == KMODE_MATERIALIZE) {

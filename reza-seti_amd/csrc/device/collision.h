@@ -25,7 +25,10 @@ struct Pop<double> {
   static constexpr bool shifted = false;
   __device__ static C ld(const void* base, int64_t i) { return static_cast<const double*>(base)[i]; }
   __device__ static void st(void* base, int64_t i, C v) { static_cast<double*>(base)[i] = v; }
-  // Nontemporal store: the value is not read again before the next step.
+  // Nontemporal (cache-bypassing) variants for streamed data.
+  __device__ static C ld_nt(const void* base, int64_t i) {
+    return __builtin_nontemporal_load(static_cast<const double*>(base) + i);
+  }
   __device__ static void st_nt(void* base, int64_t i, C v) {
     __builtin_nontemporal_store(v, static_cast<double*>(base) + i);
   }
@@ -36,6 +39,9 @@ struct Pop<float> {
   static constexpr bool shifted = true;
   __device__ static C ld(const void* base, int64_t i) { return static_cast<const float*>(base)[i]; }
   __device__ static void st(void* base, int64_t i, C v) { static_cast<float*>(base)[i] = v; }
+  __device__ static C ld_nt(const void* base, int64_t i) {
+    return __builtin_nontemporal_load(static_cast<const float*>(base) + i);
+  }
   __device__ static void st_nt(void* base, int64_t i, C v) {
     __builtin_nontemporal_store(v, static_cast<float*>(base) + i);
   }
@@ -48,6 +54,9 @@ struct Pop<__half> {
     return __half2float(static_cast<const __half*>(base)[i]);
   }
   __device__ static void st(void* base, int64_t i, C v) { static_cast<__half*>(base)[i] = __float2half_rn(v); }
+  __device__ static C ld_nt(const void* base, int64_t i) {
+    return __half2float(__ushort_as_half(__builtin_nontemporal_load(static_cast<const unsigned short*>(base) + i)));
+  }
   __device__ static void st_nt(void* base, int64_t i, C v) {
     __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(v)), static_cast<unsigned short*>(base) + i);
   }

@@ -30,6 +30,12 @@ constexpr int NCOL = 8;              // lateral column offsets
 constexpr int CHUNK = 64;            // slots per ChunkMeta (one wavefront)
 constexpr uint32_t ROW_KIND_MASK = 3u;  // NodeKind of an explicit row
 
+// Cache hints of a step's population traffic (the Infinity Cache holds one
+// FP32 state of a few million nodes but not two):
+constexpr int32_t CACHE_PLAIN = 0;     // both buffers fit: cache everything
+constexpr int32_t CACHE_NT_STORES = 1; // stream the output past the cache
+constexpr int32_t CACHE_NT_LOADS = 2;  // stream the input, keep the output cached
+
 // Lateral offset index of a direction with (cx, cy), -1 for the own column.
 __host__ __device__ constexpr int col_of(int cx, int cy) {
   return (cx == 1 && cy == 0)     ? 0
@@ -103,7 +109,7 @@ struct ColArgs {
   double outlet_rho;
   double omega;
   double force[3];
-  int32_t stream_stores;  // nontemporal population stores
+  int32_t cache_hint;     // CACHE_PLAIN, CACHE_NT_STORES or CACHE_NT_LOADS
   const float* porosity;  // per slot, or null
   const double* f_direct; // init input (FP64 unshifted, q * npad + slot), null = w_q
   double* f_out;          // materialize output

@@ -25,7 +25,7 @@ namespace {
 using namespace dev;
 using colk::ColKern;
 
-template <class L, class S, int MODE, bool FORCE, bool NT>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL>
 __device__ inline void columns_body(const ColArgs& a) {
   const int32_t b = blockIdx.x;
   if (b < a.b_blocks) {
@@ -42,12 +42,12 @@ __device__ inline void columns_body(const ColArgs& a) {
   // One wavefront covers one chunk: its metadata is wave-uniform.
   const int32_t chunk = __builtin_amdgcn_readfirstlane(p / CHUNK);
   const ChunkMeta m = a.meta[chunk];
-  if (p < a.n) ColKern<L, S>::template main_slot<MODE, FORCE, NT>(a, p, p & (CHUNK - 1), m);
+  if (p < a.n) ColKern<L, S>::template main_slot<MODE, FORCE, NT, NTL>(a, p, p & (CHUNK - 1), m);
 }
 
-template <class L, class S, int MODE, bool FORCE, bool NT>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL>
 __global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
-  columns_body<L, S, MODE, FORCE, NT>(a);
+  columns_body<L, S, MODE, FORCE, NT, NTL>(a);
 }
 
 // Occupancy-bounded variant for the float-arithmetic precisions: the
@@ -58,10 +58,10 @@ __global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
 #define LBM_COLUMNS_WAVES_PER_EU 7
 #endif
 #if LBM_COLUMNS_WAVES_PER_EU > 0
-template <class L, class S, int MODE, bool FORCE, bool NT>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL>
 __global__ __attribute__((amdgpu_waves_per_eu(LBM_COLUMNS_WAVES_PER_EU))) __launch_bounds__(BLOCK) void
 columns_kernel_occ(ColArgs a) {
-  columns_body<L, S, MODE, FORCE, NT>(a);
+  columns_body<L, S, MODE, FORCE, NT, NTL>(a);
 }
 #endif
 
@@ -96,24 +96,27 @@ __global__ __launch_bounds__(BLOCK) void wss_stats_cols(ColArgs a, double* parti
   }
 }
 
-template <class L, class S, int MODE, bool FORCE, bool NT>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL>
 hipError_t launch_one(const ColArgs& a, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>(a.b_blocks + a.m_blocks);
   if (!grid) return hipSuccess;
 #if LBM_COLUMNS_WAVES_PER_EU > 0
   if constexpr (!std::is_same<S, double>::value) {
-    hipLaunchKernelGGL((columns_kernel_occ<L, S, MODE, FORCE, NT>), dim3(grid), dim3(BLOCK), 0, st, a);
+    hipLaunchKernelGGL((columns_kernel_occ<L, S, MODE, FORCE, NT, NTL>), dim3(grid), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
   }
 #endif
-  hipLaunchKernelGGL((columns_kernel<L, S, MODE, FORCE, NT>), dim3(grid), dim3(BLOCK), 0, st, a);
+  hipLaunchKernelGGL((columns_kernel<L, S, MODE, FORCE, NT, NTL>), dim3(grid), dim3(BLOCK), 0, st, a);
   return hipGetLastError();
 }
 
 template <class L, class S, bool FORCE>
 hipError_t launch_step(const ColArgs& a, hipStream_t st) {
-  return a.stream_stores ? launch_one<L, S, KMODE_STEP, FORCE, true>(a, st)
-                         : launch_one<L, S, KMODE_STEP, FORCE, false>(a, st);
+  switch (a.cache_hint) {
+    case CACHE_NT_STORES: return launch_one<L, S, KMODE_STEP, FORCE, true, false>(a, st);
+    case CACHE_NT_LOADS: return launch_one<L, S, KMODE_STEP, FORCE, false, true>(a, st);
+  }
+  return launch_one<L, S, KMODE_STEP, FORCE, false, false>(a, st);
 }
 
 template <class L, class S>
@@ -122,10 +125,10 @@ hipError_t launch_mode(int kmode, bool force, const ColArgs& a, hipStream_t st) 
     case KMODE_STEP:
       return force ? launch_step<L, S, true>(a, st) : launch_step<L, S, false>(a, st);
     case KMODE_INIT:
-      return force ? launch_one<L, S, KMODE_INIT, true, false>(a, st)
-                   : launch_one<L, S, KMODE_INIT, false, false>(a, st);
+      return force ? launch_one<L, S, KMODE_INIT, true, false, false>(a, st)
+                   : launch_one<L, S, KMODE_INIT, false, false, false>(a, st);
     case KMODE_MATERIALIZE:
-      return launch_one<L, S, KMODE_MATERIALIZE, false, false>(a, st);
+      return launch_one<L, S, KMODE_MATERIALIZE, false, false, false>(a, st);
   }
   return hipErrorInvalidValue;
 }

@@ -123,6 +123,7 @@ struct lbm_solver {
   std::vector<int32_t> nw_rows;      // near-wall list order -> near-wall slot rank
   std::vector<int32_t> bsl_slot, bsl_kind, bsl_up, bsl_inlet;  // inlet/outlet list (bslots order)
   ColArgs cbase_args{};
+  int cache_policy = 0;              // 0 plain, 1 nontemporal stores, 2 alternating hints
 
   // Adjoint forward record (lbm_adjoint_forward).
   struct Adjoint {
@@ -195,6 +196,11 @@ ColArgs col_args_for(lbm_solver* s, int in) {
   ColArgs c = s->cbase_args;
   c.fs_in = s->fs[in];
   c.fs_out = s->fs[1 - in];
+  // Alternating policy: the step writing fs[1] keeps its output cached and
+  // streams its input; the step reading fs[1] streams its output.
+  c.cache_hint = s->cache_policy == 0 ? CACHE_PLAIN
+                 : s->cache_policy == 1 ? CACHE_NT_STORES
+                                        : (in == 0 ? CACHE_NT_LOADS : CACHE_NT_STORES);
   c.porosity = s->porosity;
   c.n_modes = s->n_modes;
   return c;
@@ -548,8 +554,16 @@ int create_columns(lbm_solver* s, const Tables& t) {
   c.outlet_rho = p.outlet_rho;
   c.omega = omega;
   for (int i = 0; i < 3; ++i) c.force[i] = p.force[i];
-  // Both buffers larger than the 256 MiB Infinity Cache: stream the stores.
-  c.stream_stores = 2 * pop_bytes > (size_t(256) << 20) ? 1 : 0;
+  // Cache policy against the 256 MiB Infinity Cache: both state buffers fit
+  // -> plain accesses; one fits -> alternate the hints so that each step's
+  // output stays cached for the next step's input (col_args_for); neither ->
+  // stream the stores.
+  const size_t mall = size_t(256) << 20;
+  s->cache_policy = 2 * pop_bytes <= mall ? 0 : (pop_bytes <= mall - mall / 8 ? 2 : 1);
+#ifdef LBM_CACHE_POLICY
+  s->cache_policy = LBM_CACHE_POLICY;  // tuning builds only
+#endif
+  c.cache_hint = CACHE_PLAIN;
 
   // Generic fields used by the shared moments kernel and init_state.
   StepArgs& a = s->base;
@@ -1229,6 +1243,7 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
 
   const bool force = has_force(s);
   ColArgs cf = cb;  // forward recomputation: no stress, no objective
+  cf.cache_hint = s->cache_policy == 0 ? CACHE_PLAIN : CACHE_NT_STORES;
   cf.wss_on = 0;
   cf.obj_acc = nullptr;
   cf.obj_w = nullptr;

@@ -82,12 +82,23 @@ struct ColKern {
     for (int q = 0; q < Q; ++q) a.f_out[int64_t(q) * a.npad + p] = SHIFT ? double(F[q]) + L::w[q] : double(F[q]);
   }
 
+  // Collide and store; near-wall node i (>= 0) also keeps the density
+  // (or its deviation) and velocity of this collision for the next step's
+  // wall shear stress.
   template <bool FORCE, bool NT>
-  __device__ static void collide_store(const ColArgs& a, int32_t p, C* F) {
+  __device__ static void collide_store(const ColArgs& a, int32_t p, C* F, int32_t i) {
     C drho, ux, uy, uz;
     const C frc[3] = {C(a.force[0]), C(a.force[1]), C(a.force[2])};
     const C alpha = (FORCE && a.porosity) ? C(a.porosity[p]) : C(0);
     collide_ext<L, C, SHIFT, FORCE>(F, C(a.omega), frc, alpha, false, drho, ux, uy, uz);
+    if (i >= 0) {
+      C* e = static_cast<C*>(a.nw_rho_u);
+      const int64_t m = a.n_nw;
+      e[i] = drho;
+      e[m + i] = ux;
+      e[2 * m + i] = uy;
+      e[3 * m + i] = uz;
+    }
     // NT: nontemporal stores (a compile-time choice: a runtime branch
     // between two otherwise identical stores is merged and loses the hint).
 #pragma unroll
@@ -101,16 +112,24 @@ struct ColKern {
   // boundary-treated populations F:
   //   tau_i = C sum_a (f_a - feq_a) (c_a.n) (c_ai - (c_a.n) n_i)
   // (Simulation.cpp:775-797 with its inner sums written out). feq is the
-  // equilibrium of the collision that produced this step's input; density
-  // and velocity are recovered from the node's own post-collision
-  // populations, which BGK and the Guo source leave with the same density
-  // and with momentum rho u (1 - alpha/2) + F/2.
+  // equilibrium of the collision that produced this step's input.
   struct Eq {
     C s, rho, ux, uy, uz;
   };
 
-  // Density and velocity of the collision that produced slot p's input
-  // (read before the pull so that only five values stay live across it).
+  // That equilibrium's density and velocity, as stored by collide_store
+  // (four independent loads, issued with the pull).
+  __device__ static Eq stored_equilibrium(const ColArgs& a, int32_t i) {
+    const C* e = static_cast<const C*>(a.nw_rho_u);
+    const int64_t m = a.n_nw;
+    const C s = e[i];
+    return Eq{s, SHIFT ? C(1) + s : s, e[m + i], e[2 * m + i], e[3 * m + i]};
+  }
+
+  // The same density and velocity recovered from the node's own
+  // post-collision populations, which BGK and the Guo source leave with the
+  // same density and with momentum rho u (1 - alpha/2) + F/2 (used by the
+  // adjoint, which differentiates through this map).
   template <bool FORCE>
   __device__ static Eq wss_equilibrium(const ColArgs& a, int32_t p) {
     C s = C(0), jx = C(0), jy = C(0), jz = C(0);
@@ -137,34 +156,35 @@ struct ColKern {
     return Eq{s, rho, jx / den, jy / den, jz / den};
   }
 
+  // The stress of this step goes to ring slot a.ring_slot (4 values in the
+  // arithmetic precision, plain stores, no dependent loads);
+  // launch_wss_flush folds the ring into the FP64 accumulators.
   __device__ static void wss_node(const ColArgs& a, int32_t i, const Eq& e, const C* F) {
+#pragma clang fp contract(fast)
     const C ux = e.ux, uy = e.uy, uz = e.uz, rho = e.rho, s = e.s;
     const C usq = ux * ux + uy * uy + uz * uz;
     const int64_t m = a.n_nw;
-    // The stress arrays are touched once per step: stream them past the
-    // caches, which are left to the populations.
-    const double n[3] = {__builtin_nontemporal_load(a.nw_normal + i), __builtin_nontemporal_load(a.nw_normal + m + i),
-                         __builtin_nontemporal_load(a.nw_normal + 2 * m + i)};
-    double tau[3] = {0.0, 0.0, 0.0};
+    const C nx = C(a.nw_normal[i]), ny = C(a.nw_normal[m + i]), nz = C(a.nw_normal[2 * m + i]);
+    C tx = C(0), ty = C(0), tz = C(0);
 #pragma unroll
     for (int al = 0; al < Q; ++al) {
-      const double fne = double(F[al] - geq_of<L, C, SHIFT>(al, SHIFT ? s : C(0), rho, ux, uy, uz, usq));
-      const double cn = L::cx[al] * n[0] + L::cy[al] * n[1] + L::cz[al] * n[2];
-      tau[0] += fne * cn * (L::cx[al] - cn * n[0]);
-      tau[1] += fne * cn * (L::cy[al] - cn * n[1]);
-      tau[2] += fne * cn * (L::cz[al] - cn * n[2]);
+      const C fne = F[al] - geq_of<L, C, SHIFT>(al, SHIFT ? s : C(0), rho, ux, uy, uz, usq);
+      const C cn = C(L::cx[al]) * nx + C(L::cy[al]) * ny + C(L::cz[al]) * nz;
+      tx += fne * cn * (C(L::cx[al]) - cn * nx);
+      ty += fne * cn * (C(L::cy[al]) - cn * ny);
+      tz += fne * cn * (C(L::cz[al]) - cn * nz);
     }
-    const double t0 = a.wss_C * tau[0], t1 = a.wss_C * tau[1], t2 = a.wss_C * tau[2];
-    const double W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
-    double* acc = a.nw_wss;
-    __builtin_nontemporal_store(W, acc + i);
-    __builtin_nontemporal_store(__builtin_nontemporal_load(acc + m + i) + t0, acc + m + i);
-    __builtin_nontemporal_store(__builtin_nontemporal_load(acc + 2 * m + i) + t1, acc + 2 * m + i);
-    __builtin_nontemporal_store(__builtin_nontemporal_load(acc + 3 * m + i) + t2, acc + 3 * m + i);
-    __builtin_nontemporal_store(__builtin_nontemporal_load(acc + 4 * m + i) + W, acc + 4 * m + i);
+    const C wc = C(a.wss_C);
+    const C t0 = wc * tx, t1 = wc * ty, t2 = wc * tz;
+    const C W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    C* r = static_cast<C*>(a.nw_ring) + int64_t(a.ring_slot) * 4 * m;
+    r[i] = t0;
+    r[m + i] = t1;
+    r[2 * m + i] = t2;
+    r[3 * m + i] = W;
     if (a.obj_acc) {  // objective: weighted sum of |tau|^2 over the chosen wall segment
       const double wt = a.obj_w[i];
-      if (wt != 0.0) a.obj_acc[i] += wt * W * W;
+      if (wt != 0.0) a.obj_acc[i] += wt * double(W) * double(W);
     }
   }
 
@@ -179,9 +199,10 @@ struct ColKern {
     const bool irr = (m.irr_mask >> l) & 1u;
     const int32_t row = irr ? m.irr_base + rank_below(m.irr_mask, l) : -1;
     if (MODE != KMODE_INIT && irr && is_boundary_kind(a, a.irr_kind[row] & ROW_KIND_MASK)) return;
-    const bool nw = MODE == KMODE_STEP && a.wss_on && ((m.nw_mask >> l) & 1u);
+    const int32_t i = ((m.nw_mask >> l) & 1u) ? m.nw_base + rank_below(m.nw_mask, l) : -1;
+    const bool nw = MODE == KMODE_STEP && a.wss_on && i >= 0;
     Eq e{};
-    if (nw) e = wss_equilibrium<FORCE>(a, p);
+    if (nw) e = stored_equilibrium(a, i);
     if (MODE == KMODE_INIT) {
       load_direct(a, p, F);
     } else if (irr) {
@@ -190,12 +211,12 @@ struct ColKern {
       const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
       pull_regular<NTL>(a, p, segment_of(m, l), bm, F);
     }
-    if (nw) wss_node(a, m.nw_base + rank_below(m.nw_mask, l), e, F);
+    if (nw) wss_node(a, i, e, F);
     if (MODE == KMODE_MATERIALIZE) {
       write_direct(a, p, F);
       return;
     }
-    collide_store<FORCE, NT>(a, p, F);
+    collide_store<FORCE, NT>(a, p, F, i);
   }
 
   __device__ static C inlet_speed(const ColArgs& a, int32_t i) {
@@ -236,9 +257,10 @@ struct ColKern {
     const int32_t row = m.irr_base + rank_below(m.irr_mask, l);
     const uint32_t kind = a.irr_kind[row] & ROW_KIND_MASK;
     const int32_t u = a.irr_bcn[row];
-    const bool nw = MODE == KMODE_STEP && a.wss_on && ((m.nw_mask >> l) & 1u);
+    const int32_t i = ((m.nw_mask >> l) & 1u) ? m.nw_base + rank_below(m.nw_mask, l) : -1;
+    const bool nw = MODE == KMODE_STEP && a.wss_on && i >= 0;
     Eq e{};
-    if (nw) e = wss_equilibrium<FORCE>(a, p);
+    if (nw) e = stored_equilibrium(a, i);
     C F[Q];
     if (kind == 2) {
       if (a.inlet_mode == INLET_NEE && u >= 0) {
@@ -272,7 +294,7 @@ struct ColKern {
       pull_irregular(a, p, row, F);
     }
     // The stress follows every boundary rule, as in the reference order.
-    if (nw) wss_node(a, m.nw_base + rank_below(m.nw_mask, l), e, F);
+    if (nw) wss_node(a, i, e, F);
     if (MODE == KMODE_STEP && a.obj_rho) {  // objective: density sums for the pressure drop
       C s = C(0);
 #pragma unroll
@@ -283,7 +305,7 @@ struct ColKern {
       write_direct(a, p, F);
       return;
     }
-    collide_store<FORCE, NT>(a, p, F);
+    collide_store<FORCE, NT>(a, p, F, i);
   }
 };
 

@@ -94,7 +94,10 @@ struct ColArgs {
   const int32_t* irr_inlet;  // n_irr: inlet basis index, or -1
   int32_t n_nw;
   const double* nw_normal;   // 3 * n_nw (SoA), near-wall slot order
-  double* nw_wss;            // 5 * n_nw: |tau|, acc tau x/y/z, acc |tau|
+  double* nw_wss;            // 5 * n_nw: |tau|, acc tau x/y/z, acc |tau| (FP64)
+  void* nw_ring;             // ring of per-step tau x/y/z, |tau| (arithmetic precision, 4 * n_nw per slot)
+  int32_t ring_slot;         // slot this step writes
+  void* nw_rho_u;            // 4 * n_nw, arithmetic precision: density (deviation), u of the last collision
   double wss_C;
   int32_t wss_on;
   const int32_t* bslots;     // inlet/outlet slots
@@ -120,6 +123,9 @@ struct ColArgs {
 };
 
 hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st);
+// Adds ring slots 0 .. count-1 (in order) to the FP64 accumulators and sets
+// the instantaneous |tau| from slot count-1.
+hipError_t launch_wss_flush(int precision, const ColArgs& a, int32_t count, hipStream_t st);
 // Sum |tau|^2 and max |tau| over near-wall nodes (2 doubles into out2);
 // partials needs 2 * ceil(n_nw / 256) doubles.
 hipError_t launch_columns_wss_stats(const ColArgs& a, double* partials, double* out2, hipStream_t st);

@@ -65,6 +65,21 @@ columns_kernel_occ(ColArgs a) {
 }
 #endif
 
+template <class C>
+__global__ __launch_bounds__(BLOCK) void wss_flush_kernel(ColArgs a, int32_t count) {
+  const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= a.n_nw) return;
+  const int64_t m = a.n_nw;
+  const C* r = static_cast<const C*>(a.nw_ring);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int32_t k = 0; k < count; ++k)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] += double(r[(int64_t(k) * 4 + c) * m + i]);
+  a.nw_wss[i] = double(r[(int64_t(count - 1) * 4 + 3) * m + i]);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) a.nw_wss[(1 + c) * m + i] += acc[c];
+}
+
 __global__ __launch_bounds__(BLOCK) void wss_stats_cols(ColArgs a, double* partials) {
   const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
   double s = 0.0, mx = 0.0;
@@ -149,6 +164,14 @@ hipError_t launch_columns(int Q, int precision, int kmode, bool force, const Col
   if (Q == 19) return launch_prec<D3Q19>(precision, kmode, force, a, st);
   if (Q == 27) return launch_prec<D3Q27>(precision, kmode, force, a, st);
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_wss_flush(int precision, const ColArgs& a, int32_t count, hipStream_t st) {
+  if (count <= 0 || a.n_nw <= 0) return hipSuccess;
+  const unsigned grid = static_cast<unsigned>((a.n_nw + BLOCK - 1) / BLOCK);
+  if (precision == 0) hipLaunchKernelGGL(wss_flush_kernel<double>, dim3(grid), dim3(BLOCK), 0, st, a, count);
+  else hipLaunchKernelGGL(wss_flush_kernel<float>, dim3(grid), dim3(BLOCK), 0, st, a, count);
+  return hipGetLastError();
 }
 
 hipError_t launch_columns_wss_stats(const ColArgs& a, double* partials, double* out2, hipStream_t st) {

@@ -119,6 +119,9 @@ struct lbm_solver {
   int32_t* bslots = nullptr;
   double* nw_normal = nullptr;
   double* nw_wss = nullptr;
+  void* nw_rho_u = nullptr;
+  void* nw_ring = nullptr;
+  int32_t ring_depth = 8, ring_count = 0;  // stress ring: slots written since the last flush
   int64_t n_irr = 0, n_nw = 0;
   std::vector<int32_t> nw_rows;      // near-wall list order -> near-wall slot rank
   std::vector<int32_t> bsl_slot, bsl_kind, bsl_up, bsl_inlet;  // inlet/outlet list (bslots order)
@@ -173,7 +176,7 @@ void free_all(lbm_solver* s) {
   void* ptrs[] = {s->fs[0], s->fs[1], s->fprev[0], s->fprev[1], s->src, s->sflags, s->bc_nbr, s->inlet_slot,
                   s->inlet_basis, s->normal, s->rho_u, s->wss, s->porosity, s->err, s->f_tmp, s->partials,
                   s->red_out, s->meta, s->seg_delta, s->bbm, s->irr_src, s->irr_kind, s->irr_bcn,
-                  s->irr_inlet, s->bslots, s->nw_normal, s->nw_wss};
+                  s->irr_inlet, s->bslots, s->nw_normal, s->nw_wss, s->nw_rho_u, s->nw_ring};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -248,6 +251,22 @@ bool has_force(const lbm_solver* s) {
   return s->porosity != nullptr || s->prm.force[0] != 0.0 || s->prm.force[1] != 0.0 || s->prm.force[2] != 0.0;
 }
 
+// Folds the pending stress ring slots into the FP64 accumulators.
+int flush_wss(lbm_solver* s) {
+  if (s->family != 1 || s->ring_count == 0) return LBM_OK;
+  HIP_TRY(launch_wss_flush(s->prec, s->cbase_args, s->ring_count, s->stream));
+  s->ring_count = 0;
+  return LBM_OK;
+}
+
+// One column-segment step with the stress ring bookkeeping.
+int column_step(lbm_solver* s, ColArgs& c, bool force) {
+  if (c.wss_on) c.ring_slot = s->ring_count;
+  HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->stream));
+  if (c.wss_on && ++s->ring_count == s->ring_depth) return flush_wss(s);
+  return LBM_OK;
+}
+
 int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
   if (nsteps < 0) return fail(LBM_ERR_ARG, "nsteps < 0");
   HIP_TRY(hipSetDevice(s->prm.device));
@@ -268,7 +287,8 @@ int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
       ColArgs c = col_args_for(s, s->cur);
       std::memcpy(c.coef, a.coef, sizeof(c.coef));
       c.wss_on = a.wss_on;
-      HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->stream));
+      int rc = column_step(s, c, force);
+      if (rc) return rc;
     } else {
       HIP_TRY(launch_lbm(s->Q, s->prec, KMODE_STEP, force, a, s->stream));
     }
@@ -489,6 +509,8 @@ int create_columns(lbm_solver* s, const Tables& t) {
       (rc = dalloc(s, &s->irr_bcn, ibcn.size())) || (rc = dalloc(s, &s->irr_inlet, iinl.size())) ||
       (rc = dalloc(s, &s->bslots, std::max<size_t>(bsl.size(), 1))) ||
       (rc = dalloc(s, &s->nw_normal, 3 * nnw1)) || (rc = dalloc(s, &s->nw_wss, 5 * nnw1)) ||
+      (rc = dalloc_raw(s, &s->nw_rho_u, 4 * nnw1 * (s->prec == 0 ? 8 : 4))) ||
+      (rc = dalloc_raw(s, &s->nw_ring, size_t(s->ring_depth) * 4 * nnw1 * (s->prec == 0 ? 8 : 4))) ||
       (rc = dalloc(s, &s->err, 1)) || (rc = dalloc(s, &s->inlet_basis, MAX_MODES * ninl)) ||
       (rc = dalloc(s, &s->partials, static_cast<size_t>(4 * nblk + 2 * ((nnw1 + BLOCK - 1) / BLOCK) + 8))) ||
       (rc = dalloc(s, &s->red_out, 8)))
@@ -505,6 +527,7 @@ int create_columns(lbm_solver* s, const Tables& t) {
       up(s->bslots, bsl.data(), bsl.size() * 4) || up(s->nw_normal, nrm.data(), nrm.size() * 8) ||
       up(s->inlet_basis, basis.data(), basis.size() * 8) ||
       hipMemsetAsync(s->nw_wss, 0, 5 * nnw1 * 8, s->stream) ||
+      hipMemsetAsync(s->nw_rho_u, 0, 4 * nnw1 * (s->prec == 0 ? 8 : 4), s->stream) ||
       hipMemsetAsync(s->err, 0, 4, s->stream) || hipMemsetAsync(s->fs[0], 0, pop_bytes, s->stream) ||
       hipMemsetAsync(s->fs[1], 0, pop_bytes, s->stream))
     return fail(LBM_ERR_HIP, "upload failed");
@@ -533,6 +556,8 @@ int create_columns(lbm_solver* s, const Tables& t) {
   c.n_nw = static_cast<int32_t>(nnw);
   c.nw_normal = s->nw_normal;
   c.nw_wss = s->nw_wss;
+  c.nw_rho_u = s->nw_rho_u;
+  c.nw_ring = s->nw_ring;
   c.wss_C = wss_C;
   c.bslots = s->bslots;
   c.n_bslots = static_cast<int32_t>(bsl.size());
@@ -964,6 +989,8 @@ int lbm_get_wss(lbm_solver* s, double* wss, double* acc_tau, double* acc_mag) {
   if (!s) return fail(LBM_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->prm.device));
   if (s->family == 1) {
+    int rc = flush_wss(s);
+    if (rc) return rc;
     const int64_t m = s->n_nw;
     std::vector<double> h(static_cast<size_t>(5 * m));
     if (m) {
@@ -1045,6 +1072,10 @@ int lbm_reduce(lbm_solver* s, lbm_stats* out) {
   HIP_TRY(hipSetDevice(s->prm.device));
   StepArgs a = args_for(s, s->cur);
   HIP_TRY(launch_moments(s->Q, s->prec, a, s->partials, s->red_out, s->stream));
+  if (s->family == 1) {
+    int rc = flush_wss(s);
+    if (rc) return rc;
+  }
   if (s->family == 1)
     HIP_TRY(launch_columns_wss_stats(s->cbase_args, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
   else
@@ -1129,7 +1160,8 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
     c.obj_acc = A.obj_acc;
     c.obj_w = A.obj_w;
     c.obj_rho = A.obj_rho;
-    HIP_TRY(launch_columns(s->Q, s->prec, KMODE_STEP, force, c, s->stream));
+    int rc = column_step(s, c, force);
+    if (rc) return rc;
     std::memcpy(s->last_coef, c.coef, sizeof(s->last_coef));
     s->cur = 1 - s->cur;
     s->iter++;

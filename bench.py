@@ -198,6 +198,47 @@ def run_adjoint(geom, info, steps=ADJ_STEPS):
     }
 
 
+C2_TAU = 0.55
+
+
+def run_c2(steps, warmup=20):
+    """D3Q27 stenosed tube (radius 64, length 1024, 50% area cosine
+    stenosis at z = 384 over 128 lu, ~12.7M nodes), parabolic NEE inlet at
+    Re 200 on the diameter, NEE zero-pressure outlet, bounce-back walls;
+    FP32-shifted and FP64 step rates with the stress computed every step."""
+    with tempfile.TemporaryDirectory() as wd:
+        cfg, info = workloads.c2_stenosis(wd, tau=C2_TAU)
+        geom = lbm.Geometry.from_config(cfg, q=27, normals=2)
+    ijk, kind, _ = geom.nodes()
+    b = geom.info.bbox
+    h = geom.info.spacing
+    inl = np.where(kind == lbm.KIND_INLET)[0]
+    x = b[0] + ijk[inl, 0] * h - info["axis"][0]
+    y = b[2] + ijk[inl, 1] * h - info["axis"][1]
+    prof = np.clip(1.0 - (x * x + y * y) / info["radius"] ** 2, 0.0, None)
+    prof /= prof.mean()
+    nu = (C2_TAU - 0.5) / 3.0
+    u_mean = 200.0 * nu / (2.0 * info["radius"])
+    out = {"workload": f"D3Q27 stenosed tube, {geom.n} nodes, parabolic inlet Re 200 (U_mean {u_mean:.4f}), "
+                       f"tau {C2_TAU}, bounce-back walls, NEE inlet/outlet", "nodes": geom.n, "steps": steps}
+    for name, prec, nbytes in (("f32", lbm.PREC_F32, 2 * 27 * 4), ("f64", lbm.PREC_F64, 2 * 27 * 8)):
+        s = lbm.Solver(geom, precision=prec, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NEE_VELOCITY,
+                       outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=C2_TAU, dt=1e-5, parity_checks=0, wss_every=1)
+        s.set_inlet_profile(prof[None, :])
+        s.step_coefficients(np.full((warmup, 1), u_mean))
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step_coefficients(np.full((steps, 1), u_mean))
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        ev_ms, _ = s.last_batch_ms()
+        mlups = geom.n * steps / wall / 1e6
+        out[f"{name}_mlups"] = round(mlups, 1)
+        out[f"{name}_roofline_frac"] = round(geom.n * nbytes / (ev_ms / 1e3 / steps) / 1e9 / PEAK_HBM_GBS, 4)
+        s.close()
+    return out
+
+
 def cpu_baseline(geom, budget_s=12.0):
     """The oracle (FP64 restatement of the reference step, OpenMP) on the host
     cores, on a bounded sample of the same workload."""
@@ -249,6 +290,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="f32,f64,f16")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--c2-steps", type=int, default=500)
     ap.add_argument("--adjoint-steps", type=int, default=ADJ_STEPS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -312,6 +355,10 @@ def main():
               f"reverse {adj['backward_s']:.3f}s", file=sys.stderr, flush=True)
         out["adjoint_over_forward"] = adj["adjoint_over_forward"]
         out["adjoint"] = adj
+    if not args.no_c2:
+        out["d3q27_stenosis"] = run_c2(args.c2_steps)
+        print(f"# D3Q27 stenosis: f32 {out['d3q27_stenosis']['f32_mlups']:.0f} MLUPS, "
+              f"f64 {out['d3q27_stenosis']['f64_mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(geom)
     print(json.dumps(out))

@@ -34,18 +34,24 @@ struct AdjArgs {
   int32_t mode;
   const void* rho_in;    // rho_{t+1} (adjoint precision, Q * npad), null at t = T
   void* rho_out;         // rho_t
+  int32_t rho_cached;    // 1: rho_in was written with cached stores (read it plainly, stream rho_out);
+                         // 0: read rho_in streamed, keep rho_out cached
   double* own;           // n_nw * Q: stress-equilibrium term (read: step t+1's, written: step t's)
-  void* mb;              // n_bslots * Q: mu_t at extrapolated inlet/outlet nodes
+  void* mb;              // (Q + 1) per member (CSR order below): mu_t of an extrapolated
+                         // inlet/outlet node and its inlet speed U_b(t)
   double* g_alpha;       // npad: dJ/dalpha accumulated over steps
   double inv_T;
   double dp_in, dp_out;  // a_p / (3 T n_in), a_p / (3 T n_out)
   const double* obj_w;   // n_nw: stress weight (0 = outside the wall segment)
   const double* g0;      // ADJ_FINAL: initial populations (FP64 unshifted, q * npad + slot), null = w_q
-  // Kernel B: upstream nodes of extrapolated boundary nodes (CSR).
+  // Kernel B: upstream nodes of extrapolated boundary nodes (CSR), with
+  // everything the transposed rules need resolved on the host.
   int32_t n_tgt;
   const int32_t* tgt;      // target slot
+  const int32_t* tgt_src;  // Q * n_tgt: raw pull source of the target (-1 = bounce back)
   const int32_t* tgt_off;  // n_tgt + 1
-  const int32_t* tgt_b;    // members: bslots index
+  const int32_t* tgt_inl;  // members: inlet basis index, or -1 for an outlet node
+  const int32_t* member_of;  // n_bslots: member position of an extrapolated node, or -1
   double* gamma;           // n_inlet: dJ/dU_b of this step (inlet basis index)
 };
 

@@ -4,6 +4,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "adjoint.h"
 #include "colkern.h"
 
@@ -72,10 +74,21 @@ struct AdjKern {
 
   __device__ static C ald(const void* b, int64_t i) { return static_cast<const C*>(b)[i]; }
   __device__ static void ast(void* b, int64_t i, C v) { static_cast<C*>(b)[i] = v; }
+  template <bool NT>
+  __device__ static C ald_h(const void* b, int64_t i) {
+    if constexpr (NT) return __builtin_nontemporal_load(static_cast<const C*>(b) + i);
+    else return static_cast<const C*>(b)[i];
+  }
+  template <bool NT>
+  __device__ static void ast_h(void* b, int64_t i, C v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, static_cast<C*>(b) + i);
+    else static_cast<C*>(b)[i] = v;
+  }
 
   // Transpose of the pull at slot p: population q of p went to the node
   // p + c_q, the pull source of p for direction opp(q); where that link is
   // missing it went to p itself as direction opp(q) (bounce-back).
+  template <bool NTIN>
   __device__ static void rpull_regular(const AdjArgs& a, int32_t p, int32_t seg, uint32_t bm, C* lam) {
     const ColArgs& c = a.c;
     const int4* d4 = reinterpret_cast<const int4*>(c.seg_delta) + 2 * int64_t(seg);
@@ -87,17 +100,18 @@ struct AdjKern {
       const int o = col_of(L::cx[qo], L::cy[qo]);
       const int32_t s = (o < 0 ? p : p + d[o]) - L::cz[qo];
       const bool bb = (bm >> qo) & 1u;
-      lam[q] = ald(a.rho_in, int64_t(bb ? qo : q) * c.npad + (bb ? p : s));
+      lam[q] = ald_h<NTIN>(a.rho_in, int64_t(bb ? qo : q) * c.npad + (bb ? p : s));
     }
   }
 
+  template <bool NTIN>
   __device__ static void rpull_irregular(const AdjArgs& a, int32_t p, int32_t row, C* lam) {
     const ColArgs& c = a.c;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int qo = L::opp[q];
       const int32_t s = c.irr_src[int64_t(qo) * c.n_irr + row];
-      lam[q] = s >= 0 ? ald(a.rho_in, int64_t(q) * c.npad + s) : ald(a.rho_in, int64_t(qo) * c.npad + p);
+      lam[q] = ald_h<NTIN>(a.rho_in, s >= 0 ? int64_t(q) * c.npad + s : int64_t(qo) * c.npad + p);
     }
   }
 
@@ -136,19 +150,15 @@ struct AdjKern {
     return double(-pu_u / (C(2) * k) - kap * rho * gu);
   }
 
-  // Stress term of the objective at near-wall node i (slot p): adds
-  // dJ_t/dG_t to mu, stores dJ_t/dh(p) (through the equilibrium recovered
-  // from the node's own input populations) for the next reverse step, and
-  // returns dJ_t/dalpha(p) of that recovery. Evaluated in the adjoint's
-  // precision, with the moments of zeta = -dJ_t/dG accumulated on the fly.
-  __device__ static double wss_adjoint(const AdjArgs& a, int32_t p, int32_t i, const C* G, C* mu) {
+  // Raw stress sum_a (G_a - feq_a) K_a (K_a = (c_a.n)(c_a - (c_a.n) n)) of
+  // near-wall node i from its recomputed populations G and the equilibrium e.
+  __device__ static void tau_raw(const ColArgs& c, int32_t i, const typename K::Eq& e, const C* G, C& tx, C& ty,
+                                 C& tz) {
 #pragma clang fp contract(fast)
-    const ColArgs& c = a.c;
-    const auto e = K::template wss_equilibrium<true>(c, p);
     const C usq = e.ux * e.ux + e.uy * e.uy + e.uz * e.uz;
     const int64_t m = c.n_nw;
     const C nx = C(c.nw_normal[i]), ny = C(c.nw_normal[m + i]), nz = C(c.nw_normal[2 * m + i]);
-    C tx = C(0), ty = C(0), tz = C(0);
+    tx = ty = tz = C(0);
 #pragma unroll
     for (int al = 0; al < Q; ++al) {
       const C fne = G[al] - geq_of<L, C, SHIFT>(al, SHIFT ? e.s : C(0), e.rho, e.ux, e.uy, e.uz, usq);
@@ -157,6 +167,19 @@ struct AdjKern {
       ty += fne * cn * (C(L::cy[al]) - cn * ny);
       tz += fne * cn * (C(L::cz[al]) - cn * nz);
     }
+  }
+
+  // Stress term of the objective at near-wall node i (slot p), given its raw
+  // stress (tx, ty, tz) and the equilibrium e recovered from the node's own
+  // input populations: adds dJ_t/dG_t to mu, stores dJ_t/dh(p) (through that
+  // recovery) for the next reverse step, and returns dJ_t/dalpha(p) of the
+  // recovery. The moments of zeta = -dJ_t/dG are accumulated on the fly.
+  __device__ static double wss_adjoint(const AdjArgs& a, int32_t p, int32_t i, const typename K::Eq& e, C tx, C ty,
+                                       C tz, C* mu) {
+#pragma clang fp contract(fast)
+    const ColArgs& c = a.c;
+    const int64_t m = c.n_nw;
+    const C nx = C(c.nw_normal[i]), ny = C(c.nw_normal[m + i]), nz = C(c.nw_normal[2 * m + i]);
     // J_t = w C^2 |tau_raw|^2 / T  ->  dJ_t/dG_a = 2 w C^2 / T (tau_raw . K_a)
     const C coef = C(2.0 * a.obj_w[i] * a.inv_T * c.wss_C * c.wss_C);
     WMom<L, C> M;
@@ -198,12 +221,15 @@ struct AdjKern {
     for (int q = 0; q < Q; ++q) lam[q] += C(a.own[int64_t(i) * Q + q]);
   }
 
+  // The recomputed state is read once per reverse step: stream it past the
+  // caches, which keep the adjoint field written by the previous launch.
   __device__ static void pull_main(const AdjArgs& a, int32_t p, bool irr, int32_t row, int32_t seg, uint32_t bm,
                                    C* G) {
-    if (irr) K::pull_irregular(a.c, p, row, G);
-    else K::pull_regular(a.c, p, seg, bm, G);
+    if (irr) K::template pull_irregular<true>(a.c, p, row, G);
+    else K::template pull_regular<true>(a.c, p, seg, bm, G);
   }
 
+  template <bool NTIN, bool NTOUT>
   __device__ static void main_slot(const AdjArgs& a, int32_t p, int l, const ChunkMeta& m) {
     const ColArgs& c = a.c;
     const bool irr = (m.irr_mask >> l) & 1u;
@@ -231,8 +257,8 @@ struct AdjKern {
     }
     C lam[Q];
     if (a.rho_in) {
-      if (irr) rpull_irregular(a, p, row, lam);
-      else rpull_regular(a, p, seg, bm, lam);
+      if (irr) rpull_irregular<NTIN>(a, p, row, lam);
+      else rpull_regular<NTIN>(a, p, seg, bm, lam);
       if (in_obj) add_own(a, i, lam);
     } else {
 #pragma unroll
@@ -240,14 +266,19 @@ struct AdjKern {
     }
     double dal = collide_adjoint(a, p, s, jx, jy, jz, lam);  // lam -> mu
     if (a.mode == ADJ_STEP && in_obj) {
-      C G[Q];
-      pull_main(a, p, irr, row, seg, bm, G);
-      dal += wss_adjoint(a, p, i, G, lam);
+      const auto e = K::template wss_equilibrium<true>(c, p);
+      C tx, ty, tz;
+      {
+        C G[Q];
+        pull_main(a, p, irr, row, seg, bm, G);
+        tau_raw(c, i, e, G, tx, ty, tz);
+      }
+      dal += wss_adjoint(a, p, i, e, tx, ty, tz, lam);
     }
     if (a.g_alpha) a.g_alpha[p] += dal;
     if (a.mode == ADJ_FINAL) return;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) ast(a.rho_out, int64_t(q) * c.npad + p, lam[q]);
+    for (int q = 0; q < Q; ++q) ast_h<NTOUT>(a.rho_out, int64_t(q) * c.npad + p, lam[q]);
   }
 
   // Post-rule populations of inlet/outlet node p (as the forward step).
@@ -288,7 +319,7 @@ struct AdjKern {
     }
     C lam[Q];
     if (a.rho_in) {
-      rpull_irregular(a, p, row, lam);
+      rpull_irregular<false>(a, p, row, lam);
       if (in_obj) add_own(a, i, lam);
     } else {
 #pragma unroll
@@ -296,19 +327,27 @@ struct AdjKern {
     }
     double dal = collide_adjoint(a, p, s, jx, jy, jz, lam);  // lam -> mu
     if (in_obj) {
-      C G[Q];
-      boundary_G(c, p, row, kind, u, nee, G);
-      dal += wss_adjoint(a, p, i, G, lam);
+      const auto e = K::template wss_equilibrium<true>(c, p);
+      C tx, ty, tz;
+      {
+        C G[Q];
+        boundary_G(c, p, row, kind, u, nee, G);
+        tau_raw(c, i, e, G, tx, ty, tz);
+      }
+      dal += wss_adjoint(a, p, i, e, tx, ty, tz, lam);
     }
     const C dp = kind == 2 ? C(a.dp_in) : C(-a.dp_out);
 #pragma unroll
     for (int q = 0; q < Q; ++q) lam[q] += dp;
     if (a.g_alpha) a.g_alpha[p] += dal;
+    if (nee) {
+      const int64_t j = a.member_of[bi];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      if (nee) ast(a.mb, int64_t(bi) * Q + q, lam[q]);
-      ast(a.rho_out, int64_t(q) * c.npad + p, nee ? C(0) : lam[q]);
+      for (int q = 0; q < Q; ++q) ast(a.mb, j * (Q + 1) + q, lam[q]);
+      ast(a.mb, j * (Q + 1) + Q, kind == 2 ? K::inlet_speed(c, c.irr_inlet[row]) : C(0));
     }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) ast(a.rho_out, int64_t(q) * c.npad + p, nee ? C(0) : lam[q]);
   }
 
   // Upstream node y of extrapolated boundary nodes b: adds NEE_b'^T mu(b) to
@@ -324,7 +363,11 @@ struct AdjKern {
     const ColArgs& c = a.c;
     const int32_t y = a.tgt[k];
     C R[Q];
-    K::pull_any(c, y, R);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int32_t s = a.tgt_src[int64_t(k) * Q + q];
+      R[q] = K::P::ld(c.fs_in, int64_t(s >= 0 ? q : L::opp[q]) * c.npad + (s >= 0 ? s : y));
+    }
     C s, jx, jy, jz;
     moments<L, C>(R, s, jx, jy, jz);
     const C rR = SHIFT ? C(1) + s : s;
@@ -333,24 +376,18 @@ struct AdjKern {
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = C(0);
     for (int32_t j = a.tgt_off[k]; j < a.tgt_off[k + 1]; ++j) {
-      const int32_t bi = a.tgt_b[j];
-      const int32_t pb = c.bslots[bi];
-      const ChunkMeta mb = c.meta[pb / CHUNK];
-      const int lb = pb & (CHUNK - 1);
-      const int32_t row = mb.irr_base + rank_below(mb.irr_mask, lb);
-      const uint32_t kind = c.irr_kind[row] & ROW_KIND_MASK;
+      const int32_t inl = a.tgt_inl[j];
       C mu[Q];
 #pragma unroll
-      for (int q = 0; q < Q; ++q) mu[q] = ald(a.mb, int64_t(bi) * Q + q);
+      for (int q = 0; q < Q; ++q) mu[q] = ald(a.mb, int64_t(j) * (Q + 1) + q);
       WMom<L, C> M;
       M.of(mu);
       const C psiR = M.psi(ux, uy, uz);
       C gx, gy, gz;
       M.grad(ux, uy, uz, gx, gy, gz);
       C cst, fac;
-      if (kind == 2) {
-        const int32_t inl = c.irr_inlet[row];
-        const C U = K::inlet_speed(c, inl);
+      if (inl >= 0) {
+        const C U = ald(a.mb, int64_t(j) * (Q + 1) + Q);
         const C nx = C(c.inlet_normal[0]), ny = C(c.inlet_normal[1]), nz = C(c.inlet_normal[2]);
         const C bx = -nx * U, by = -ny * U, bz = -nz * U;
         cst = M.psi(bx, by, bz) - psiR;
@@ -376,7 +413,7 @@ struct AdjKern {
 };
 
 template <class L, class S>
-__global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
+__device__ inline void adjoint_body(const AdjArgs& a) {
   const ColArgs& c = a.c;
   const int32_t b = blockIdx.x;
   if (b < c.b_blocks) {
@@ -392,12 +429,31 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
   const int32_t p = lb * BLOCK + threadIdx.x;
   const int32_t chunk = __builtin_amdgcn_readfirstlane(p / CHUNK);
   const ChunkMeta m = c.meta[chunk];
-  if (p < c.n) AdjKern<L, S>::main_slot(a, p, p & (CHUNK - 1), m);
+  if (p >= c.n) return;
+  if (a.rho_cached) AdjKern<L, S>::template main_slot<false, true>(a, p, p & (CHUNK - 1), m);
+  else AdjKern<L, S>::template main_slot<true, false>(a, p, p & (CHUNK - 1), m);
 }
 
 template <class L, class S>
-__global__ __launch_bounds__(BLOCK) void adjoint_boundary_kernel(AdjArgs a) {
-  const int32_t k = blockIdx.x * BLOCK + threadIdx.x;
+__global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
+  adjoint_body<L, S>(a);
+}
+
+#if LBM_ADJOINT_WAVES_PER_EU > 0
+template <class L, class S>
+__global__ __attribute__((amdgpu_waves_per_eu(LBM_ADJOINT_WAVES_PER_EU))) __launch_bounds__(BLOCK) void
+adjoint_kernel_occ(AdjArgs a) {
+  adjoint_body<L, S>(a);
+}
+#endif
+
+// One wavefront per block: the few thousand upstream nodes spread over as
+// many CUs as possible (the kernel is latency-bound).
+constexpr int BBLOCK = 64;
+
+template <class L, class S>
+__global__ __launch_bounds__(BBLOCK) void adjoint_boundary_kernel(AdjArgs a) {
+  const int32_t k = blockIdx.x * BBLOCK + threadIdx.x;
   if (k < a.n_tgt) AdjKern<L, S>::upstream(a, k);
 }
 
@@ -425,15 +481,22 @@ __global__ __launch_bounds__(BLOCK) void coef_gradient_kernel(const double* gamm
 template <class L, class S>
 hipError_t launch_a(const AdjArgs& a, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>(a.c.b_blocks + a.c.m_blocks);
-  if (grid) hipLaunchKernelGGL((adjoint_kernel<L, S>), dim3(grid), dim3(BLOCK), 0, st, a);
+  if (!grid) return hipSuccess;
+#if LBM_ADJOINT_WAVES_PER_EU > 0
+  if constexpr (!std::is_same<S, double>::value) {
+    hipLaunchKernelGGL((adjoint_kernel_occ<L, S>), dim3(grid), dim3(BLOCK), 0, st, a);
+    return hipGetLastError();
+  }
+#endif
+  hipLaunchKernelGGL((adjoint_kernel<L, S>), dim3(grid), dim3(BLOCK), 0, st, a);
   return hipGetLastError();
 }
 
 template <class L, class S>
 hipError_t launch_b(const AdjArgs& a, hipStream_t st) {
   if (a.n_tgt > 0)
-    hipLaunchKernelGGL((adjoint_boundary_kernel<L, S>), dim3((a.n_tgt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st,
-                       a);
+    hipLaunchKernelGGL((adjoint_boundary_kernel<L, S>), dim3((a.n_tgt + BBLOCK - 1) / BBLOCK), dim3(BBLOCK), 0,
+                       st, a);
   return hipGetLastError();
 }
 

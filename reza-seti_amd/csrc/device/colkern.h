@@ -159,7 +159,7 @@ struct ColKern {
   // The stress of this step goes to ring slot a.ring_slot (4 values in the
   // arithmetic precision, plain stores, no dependent loads);
   // launch_wss_flush folds the ring into the FP64 accumulators.
-  __device__ static void wss_node(const ColArgs& a, int32_t i, const Eq& e, const C* F) {
+  __device__ static void wss_node(const ColArgs& a, int32_t i, const Eq& e, const C* F, double wt) {
 #pragma clang fp contract(fast)
     const C ux = e.ux, uy = e.uy, uz = e.uz, rho = e.rho, s = e.s;
     const C usq = ux * ux + uy * uy + uz * uz;
@@ -182,10 +182,9 @@ struct ColKern {
     r[m + i] = t1;
     r[2 * m + i] = t2;
     r[3 * m + i] = W;
-    if (a.obj_acc) {  // objective: weighted sum of |tau|^2 over the chosen wall segment
-      const double wt = a.obj_w[i];
-      if (wt != 0.0) a.obj_acc[i] += wt * double(W) * double(W);
-    }
+    // Objective: weighted sum of |tau|^2 over the chosen wall segment (wt
+    // was loaded with the pull; zero outside the segment or the adjoint).
+    if (wt != 0.0) a.obj_acc[i] += wt * double(W) * double(W);
   }
 
   __device__ static bool is_boundary_kind(const ColArgs& a, uint32_t kind) {
@@ -202,7 +201,11 @@ struct ColKern {
     const int32_t i = ((m.nw_mask >> l) & 1u) ? m.nw_base + rank_below(m.nw_mask, l) : -1;
     const bool nw = MODE == KMODE_STEP && a.wss_on && i >= 0;
     Eq e{};
-    if (nw) e = stored_equilibrium(a, i);
+    double wt = 0.0;
+    if (nw) {
+      e = stored_equilibrium(a, i);
+      if (a.obj_acc) wt = a.obj_w[i];
+    }
     if (MODE == KMODE_INIT) {
       load_direct(a, p, F);
     } else if (irr) {
@@ -211,7 +214,7 @@ struct ColKern {
       const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
       pull_regular<NTL>(a, p, segment_of(m, l), bm, F);
     }
-    if (nw) wss_node(a, i, e, F);
+    if (nw) wss_node(a, i, e, F, wt);
     if (MODE == KMODE_MATERIALIZE) {
       write_direct(a, p, F);
       return;
@@ -219,9 +222,12 @@ struct ColKern {
     collide_store<FORCE, NT>(a, p, F, i);
   }
 
+  // Unrolled to the fixed mode bound so the basis loads issue together.
   __device__ static C inlet_speed(const ColArgs& a, int32_t i) {
     double u = 0.0;
-    for (int m = 0; m < a.n_modes; ++m) u += a.coef[m] * a.inlet_basis[int64_t(m) * a.n_inlet + i];
+#pragma unroll
+    for (int m = 0; m < MAX_MODES; ++m)
+      if (m < a.n_modes) u += a.coef[m] * a.inlet_basis[int64_t(m) * a.n_inlet + i];
     return C(u);
   }
 
@@ -260,7 +266,11 @@ struct ColKern {
     const int32_t i = ((m.nw_mask >> l) & 1u) ? m.nw_base + rank_below(m.nw_mask, l) : -1;
     const bool nw = MODE == KMODE_STEP && a.wss_on && i >= 0;
     Eq e{};
-    if (nw) e = stored_equilibrium(a, i);
+    double wt = 0.0;
+    if (nw) {
+      e = stored_equilibrium(a, i);
+      if (a.obj_acc) wt = a.obj_w[i];
+    }
     C F[Q];
     if (kind == 2) {
       if (a.inlet_mode == INLET_NEE && u >= 0) {
@@ -294,7 +304,7 @@ struct ColKern {
       pull_irregular(a, p, row, F);
     }
     // The stress follows every boundary rule, as in the reference order.
-    if (nw) wss_node(a, i, e, F);
+    if (nw) wss_node(a, i, e, F, wt);
     if (MODE == KMODE_STEP && a.obj_rho) {  // objective: density sums for the pressure drop
       C s = C(0);
 #pragma unroll

@@ -125,8 +125,25 @@ struct lbm_solver {
   int64_t n_irr = 0, n_nw = 0;
   std::vector<int32_t> nw_rows;      // near-wall list order -> near-wall slot rank
   std::vector<int32_t> bsl_slot, bsl_kind, bsl_up, bsl_inlet;  // inlet/outlet list (bslots order)
+  std::vector<int32_t> bsl_up_src;  // Q per list entry: raw pull sources of the node behind (-1 = bounce back)
   ColArgs cbase_args{};
   int cache_policy = 0;              // 0 plain, 1 nontemporal stores, 2 alternating hints
+
+  // Reverse-pass workspace, kept across lbm_adjoint_backward calls (freed
+  // with the solver): state slots in one arena, the two adjoint fields, and
+  // the upstream-node tables of the transposed boundary rules.
+  struct AdjWork {
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
+    char* ckpt_arena = nullptr;  // checkpoints of lbm_adjoint_forward
+    size_t ckpt_bytes = 0;
+    void* rho[2] = {nullptr, nullptr};
+    void* mb = nullptr;
+    double* own = nullptr;
+    double* galpha = nullptr;
+    int32_t *tgt = nullptr, *tgt_src = nullptr, *tgt_off = nullptr, *tgt_inl = nullptr, *member_of = nullptr;
+    int32_t n_tgt = -1;  // -1: tables not built yet
+  } adjw;
 
   // Adjoint forward record (lbm_adjoint_forward).
   struct Adjoint {
@@ -163,9 +180,8 @@ int dalloc_raw(lbm_solver* s, void** p, size_t bytes) {
 }
 
 void adjoint_release(lbm_solver* s) {
-  for (void* p : s->adj.ckpt)
-    if (p) (void)hipFree(p);
-  void* ptrs[] = {s->adj.prev, s->adj.obj_w, s->adj.obj_acc, s->adj.obj_rho};
+  // The checkpoints live in the solver's persistent arena.
+  void* ptrs[] = {s->adj.obj_w, s->adj.obj_acc, s->adj.obj_rho};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   s->adj = lbm_solver::Adjoint{};
@@ -173,6 +189,14 @@ void adjoint_release(lbm_solver* s) {
 
 void free_all(lbm_solver* s) {
   adjoint_release(s);
+  {
+    auto& W = s->adjw;
+    void* ptrs[] = {W.arena, W.ckpt_arena, W.rho[0], W.rho[1], W.mb, W.own, W.galpha, W.tgt, W.tgt_src, W.tgt_off,
+                    W.tgt_inl, W.member_of};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    W = lbm_solver::AdjWork{};
+  }
   void* ptrs[] = {s->fs[0], s->fs[1], s->fprev[0], s->fprev[1], s->src, s->sflags, s->bc_nbr, s->inlet_slot,
                   s->inlet_basis, s->normal, s->rho_u, s->wss, s->porosity, s->err, s->f_tmp, s->partials,
                   s->red_out, s->meta, s->seg_delta, s->bbm, s->irr_src, s->irr_kind, s->irr_bcn,
@@ -565,10 +589,13 @@ int create_columns(lbm_solver* s, const Tables& t) {
   s->bsl_kind.resize(bsl.size());
   s->bsl_up.resize(bsl.size());
   s->bsl_inlet.resize(bsl.size());
+  s->bsl_up_src.assign(bsl.size() * Q, -1);
   for (size_t i = 0; i < bsl.size(); ++i) {
     s->bsl_kind[i] = t.kind[bsl[i]];
     s->bsl_up[i] = t.bc_nbr[bsl[i]];
     s->bsl_inlet[i] = inlet_index[bsl[i]];
+    if (s->bsl_up[i] >= 0)
+      for (int q = 0; q < Q; ++q) s->bsl_up_src[i * Q + q] = srcq(q, s->bsl_up[i]);
   }
   c.n_inlet = static_cast<int32_t>(ninl);
   c.n_modes = 1;
@@ -1129,20 +1156,31 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
   }
   A.wss_weight = obj->wss_weight;
   A.dp_weight = obj->dp_weight;
-  A.g0 = s->last_set;
+  // The initial populations move to the record: lbm_get_populations only
+  // reads them before the first step, and the forward pass steps.
+  A.g0 = std::move(s->last_set);
+  s->last_set.clear();
   A.T = T;
   A.K = K;
   A.nck = (T + K - 1) / K + 1;
   A.ckpt.assign(A.nck, nullptr);
-  if (hipMalloc(&A.prev, state_bytes) != hipSuccess) {
-    adjoint_release(s);
-    return fail(LBM_ERR_HIP, "not enough device memory for the checkpoints");
-  }
-  for (auto& p : A.ckpt) {
-    if (hipMalloc(&p, state_bytes) != hipSuccess) {
-      adjoint_release(s);
-      return fail(LBM_ERR_HIP, "not enough device memory for the checkpoints");
+  // Checkpoints h_0, h_K, ..., h_T and the last step's input h_{T-1}, in a
+  // persistent arena (reused by later forward passes of the same size).
+  {
+    auto& W = s->adjw;
+    const size_t need = (static_cast<size_t>(A.nck) + 1) * state_bytes;
+    if (W.ckpt_bytes < need) {
+      if (W.ckpt_arena) (void)hipFree(W.ckpt_arena);
+      W.ckpt_arena = nullptr;
+      W.ckpt_bytes = 0;
+      if (hipMalloc(reinterpret_cast<void**>(&W.ckpt_arena), need) != hipSuccess) {
+        adjoint_release(s);
+        return fail(LBM_ERR_HIP, "not enough device memory for the checkpoints");
+      }
+      W.ckpt_bytes = need;
     }
+    for (int32_t i = 0; i < A.nck; ++i) A.ckpt[i] = W.ckpt_arena + static_cast<size_t>(i) * state_bytes;
+    A.prev = W.ckpt_arena + static_cast<size_t>(A.nck) * state_bytes;
   }
   A.coef.assign(static_cast<size_t>(T) * M, 0.0);
   HIP_TRY(hipMemcpyAsync(A.ckpt[0], s->fs[s->cur], state_bytes, hipMemcpyDeviceToDevice, s->stream));
@@ -1205,21 +1243,22 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
   ColArgs cb = col_args_for(s, 0);
   const int32_t n_inlet = cb.n_inlet;
 
-  AdjointScratch S;
-  void *rho0 = nullptr, *rho1 = nullptr, *mb = nullptr, *scratch = nullptr;
-  double *own = nullptr, *galpha = nullptr, *gamma = nullptr, *g0 = nullptr, *gcoef = nullptr;
-  int32_t *tgt = nullptr, *tgt_off = nullptr, *tgt_b = nullptr;
-  HIP_TRY(S.alloc(&rho0, static_cast<size_t>(Q) * npad * aes));
-  HIP_TRY(S.alloc(&rho1, static_cast<size_t>(Q) * npad * aes));
-  HIP_TRY(S.alloc(&own, nnw * Q * 8));
-  HIP_TRY(S.alloc(&mb, nb * Q * aes));
-  HIP_TRY(S.alloc(&galpha, static_cast<size_t>(npad) * 8));
+  AdjointScratch S;  // per-call buffers (their sizes depend on the forward pass)
+  auto& W = s->adjw;
+  auto keep = [&](auto** p, size_t bytes) -> hipError_t {
+    if (*p) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(bytes, 8));
+  };
+  HIP_TRY(keep(&W.rho[0], static_cast<size_t>(Q) * npad * aes));
+  HIP_TRY(keep(&W.rho[1], static_cast<size_t>(Q) * npad * aes));
+  HIP_TRY(keep(&W.own, nnw * Q * 8));
+  HIP_TRY(keep(&W.mb, nb * (Q + 1) * aes));
+  HIP_TRY(keep(&W.galpha, static_cast<size_t>(npad) * 8));
+  double *gamma = nullptr, *g0 = nullptr, *gcoef = nullptr;
   HIP_TRY(S.alloc(&gamma, static_cast<size_t>(T) * n_inlet * 8));
   HIP_TRY(S.alloc(&gcoef, static_cast<size_t>(T) * std::max(M, 1) * 8));
-  HIP_TRY(hipMemsetAsync(galpha, 0, static_cast<size_t>(npad) * 8, s->stream));
+  HIP_TRY(hipMemsetAsync(W.galpha, 0, static_cast<size_t>(npad) * 8, s->stream));
   HIP_TRY(hipMemsetAsync(gamma, 0, static_cast<size_t>(T) * n_inlet * 8, s->stream));
-  HIP_TRY(hipMemsetAsync(rho0, 0, static_cast<size_t>(Q) * npad * aes, s->stream));
-  HIP_TRY(hipMemsetAsync(rho1, 0, static_cast<size_t>(Q) * npad * aes, s->stream));
   if (!A.g0.empty()) {
     std::vector<double> soa(static_cast<size_t>(Q) * npad, 0.0);
     const lbm::LatticeView L = lbm::lattice_view(Q);
@@ -1232,46 +1271,71 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
     HIP_TRY(hipStreamSynchronize(s->stream));
   }
 
-  // Upstream nodes of extrapolated inlet/outlet nodes, grouped (CSR).
-  std::vector<std::pair<int32_t, int32_t>> up;  // (upstream slot, bslots index)
-  for (size_t i = 0; i < s->bsl_slot.size(); ++i) {
-    const int32_t u = s->bsl_up[i], k = s->bsl_kind[i];
-    const bool nee = u >= 0 && ((k == KIND_INLET && s->prm.inlet_mode == LBM_INLET_NEE_VELOCITY) ||
-                                (k == KIND_OUTLET && s->prm.outlet_mode == LBM_OUTLET_NEE_PRESSURE));
-    if (nee) up.emplace_back(s->slot_of[u], static_cast<int32_t>(i));
-  }
-  std::sort(up.begin(), up.end());
-  std::vector<int32_t> h_tgt, h_off, h_b;
-  for (size_t i = 0; i < up.size(); ++i) {
-    if (i == 0 || up[i].first != up[i - 1].first) {
-      h_tgt.push_back(up[i].first);
-      h_off.push_back(static_cast<int32_t>(i));
+  // Upstream nodes of extrapolated inlet/outlet nodes, grouped (CSR), with
+  // their raw pull sources and inlet indices (built once per solver).
+  if (W.n_tgt < 0) {
+    std::vector<std::pair<int32_t, int32_t>> up;  // (upstream slot, bslots index)
+    for (size_t i = 0; i < s->bsl_slot.size(); ++i) {
+      const int32_t u = s->bsl_up[i], k = s->bsl_kind[i];
+      const bool nee = u >= 0 && ((k == KIND_INLET && s->prm.inlet_mode == LBM_INLET_NEE_VELOCITY) ||
+                                  (k == KIND_OUTLET && s->prm.outlet_mode == LBM_OUTLET_NEE_PRESSURE));
+      if (nee) up.emplace_back(s->slot_of[u], static_cast<int32_t>(i));
     }
-    h_b.push_back(up[i].second);
+    std::sort(up.begin(), up.end());
+    std::vector<int32_t> h_tgt, h_off, h_b, h_inl, h_src;
+    for (size_t i = 0; i < up.size(); ++i) {
+      if (i == 0 || up[i].first != up[i - 1].first) {
+        h_tgt.push_back(up[i].first);
+        h_off.push_back(static_cast<int32_t>(i));
+      }
+      h_b.push_back(up[i].second);
+      h_inl.push_back(s->bsl_kind[up[i].second] == KIND_INLET ? s->bsl_inlet[up[i].second] : -1);
+    }
+    h_off.push_back(static_cast<int32_t>(up.size()));
+    h_src.resize(h_tgt.size() * Q);
+    for (size_t k = 0; k < h_tgt.size(); ++k)
+      for (int q = 0; q < Q; ++q) h_src[k * Q + q] = s->bsl_up_src[size_t(h_b[h_off[k]]) * Q + q];
+    std::vector<int32_t> h_member(std::max<size_t>(s->bsl_slot.size(), 1), -1);
+    for (size_t j = 0; j < h_b.size(); ++j) h_member[h_b[j]] = static_cast<int32_t>(j);
+    HIP_TRY(keep(&W.tgt, h_tgt.size() * 4));
+    HIP_TRY(keep(&W.tgt_off, h_off.size() * 4));
+    HIP_TRY(keep(&W.tgt_inl, h_inl.size() * 4));
+    HIP_TRY(keep(&W.tgt_src, h_src.size() * 4));
+    HIP_TRY(keep(&W.member_of, h_member.size() * 4));
+    HIP_TRY(hipMemcpyAsync(W.member_of, h_member.data(), h_member.size() * 4, hipMemcpyHostToDevice, s->stream));
+    if (!h_tgt.empty()) {
+      HIP_TRY(hipMemcpyAsync(W.tgt, h_tgt.data(), h_tgt.size() * 4, hipMemcpyHostToDevice, s->stream));
+      HIP_TRY(hipMemcpyAsync(W.tgt_inl, h_inl.data(), h_inl.size() * 4, hipMemcpyHostToDevice, s->stream));
+      HIP_TRY(hipMemcpyAsync(W.tgt_src, h_src.data(), h_src.size() * 4, hipMemcpyHostToDevice, s->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(W.tgt_off, h_off.data(), h_off.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    W.n_tgt = static_cast<int32_t>(h_tgt.size());
   }
-  h_off.push_back(static_cast<int32_t>(up.size()));
-  HIP_TRY(S.alloc(&tgt, h_tgt.size() * 4));
-  HIP_TRY(S.alloc(&tgt_off, h_off.size() * 4));
-  HIP_TRY(S.alloc(&tgt_b, h_b.size() * 4));
-  if (!h_tgt.empty()) {
-    HIP_TRY(hipMemcpyAsync(tgt, h_tgt.data(), h_tgt.size() * 4, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(tgt_b, h_b.data(), h_b.size() * 4, hipMemcpyHostToDevice, s->stream));
-  }
-  HIP_TRY(hipMemcpyAsync(tgt_off, h_off.data(), h_off.size() * 4, hipMemcpyHostToDevice, s->stream));
 
-  // State buffers for the binomial schedule.
+  // State buffers for the binomial schedule: `slots` of them plus a
+  // scratch state, in one arena that later calls reuse.
   const int32_t max_slots = std::max<int32_t>(K - 1, 0);
   if (slots <= 0 || slots > max_slots) {
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const size_t reserve = size_t(1) << 30;
-    const int64_t fit = free_b > reserve ? static_cast<int64_t>((free_b - reserve) / state_bytes) : 0;
+    const size_t avail = free_b + W.arena_bytes;
+    const int64_t fit = avail > reserve ? static_cast<int64_t>((avail - reserve) / state_bytes) - 1 : 0;
     const int32_t want = slots <= 0 ? max_slots : std::min(slots, max_slots);
-    slots = static_cast<int32_t>(std::min<int64_t>(want, fit));
+    slots = static_cast<int32_t>(std::max<int64_t>(0, std::min<int64_t>(want, fit)));
+  }
+  const size_t need = (static_cast<size_t>(slots) + 1) * state_bytes;
+  if (W.arena_bytes < need) {
+    if (W.arena) (void)hipFree(W.arena);
+    W.arena = nullptr;
+    W.arena_bytes = 0;
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&W.arena), need));
+    W.arena_bytes = need;
   }
   std::vector<void*> slot_buf(slots, nullptr);
-  for (auto& p : slot_buf) HIP_TRY(S.alloc(&p, state_bytes));
-  HIP_TRY(S.alloc(&scratch, state_bytes));
+  for (int32_t i = 0; i < slots; ++i) slot_buf[i] = W.arena + static_cast<size_t>(i) * state_bytes;
+  void* scratch = W.arena + static_cast<size_t>(slots) * state_bytes;
 
   const bool force = has_force(s);
   ColArgs cf = cb;  // forward recomputation: no stress, no objective
@@ -1296,18 +1360,20 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
   a.c = cb;
   a.c.obj_acc = nullptr;
   a.c.obj_rho = nullptr;
-  a.own = own;
-  a.mb = mb;
-  a.g_alpha = galpha;
+  a.own = W.own;
+  a.mb = W.mb;
+  a.g_alpha = W.galpha;
   a.inv_T = 1.0 / T;
   a.dp_in = A.n_in ? A.dp_weight / (3.0 * T * A.n_in) : 0.0;
   a.dp_out = A.n_out ? A.dp_weight / (3.0 * T * A.n_out) : 0.0;
   a.obj_w = A.obj_w;
-  a.n_tgt = static_cast<int32_t>(h_tgt.size());
-  a.tgt = tgt;
-  a.tgt_off = tgt_off;
-  a.tgt_b = tgt_b;
-  void* rho[2] = {rho0, rho1};
+  a.n_tgt = W.n_tgt;
+  a.tgt = W.tgt;
+  a.tgt_src = W.tgt_src;
+  a.tgt_off = W.tgt_off;
+  a.tgt_inl = W.tgt_inl;
+  a.member_of = W.member_of;
+  void* rho[2] = {W.rho[0], W.rho[1]};
   auto reverse = [&](int32_t t, void* h) -> int {
     AdjArgs x = a;
     x.mode = ADJ_STEP;
@@ -1315,6 +1381,11 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
     for (int m = 0; m < MAX_MODES; ++m) x.c.coef[m] = m < M ? A.coef[int64_t(t - 1) * M + m] : 0.0;
     x.rho_in = t == T ? nullptr : rho[(t + 1) & 1];
     x.rho_out = rho[t & 1];
+    // Alternate the cache hints so that every other launch reads an adjoint
+    // field still held in the Infinity Cache: odd steps write rho[1] with
+    // cached stores (and stream their input), even steps read it plainly
+    // (and stream their output).
+    x.rho_cached = (t & 1) ? 0 : 1;
     x.gamma = gamma + int64_t(t - 1) * n_inlet;
     HIP_TRY(launch_adjoint(Q, s->prec, x, s->stream));
     HIP_TRY(launch_adjoint_boundary(Q, s->prec, x, s->stream));
@@ -1347,7 +1418,7 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
   std::vector<double> ga;
   if (grad_porosity) {
     ga.resize(npad);
-    HIP_TRY(hipMemcpyAsync(ga.data(), galpha, static_cast<size_t>(npad) * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(ga.data(), W.galpha, static_cast<size_t>(npad) * 8, hipMemcpyDeviceToHost, s->stream));
   }
   // Restore the forward pass's last two states (the current state and the
   // input of the last step, which lbm_get_populations re-streams).

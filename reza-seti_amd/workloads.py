@@ -112,6 +112,68 @@ def c0_tube(workdir, radius=10.0, length=80.0, tau=0.8, cte_vel=0.05, max_iter=2
     return cfg
 
 
+def revolution_surface(radius_of_z, z_levels, segments=128, center=(0.37, 0.21)):
+    """Closed surface of revolution about the z axis through `center`:
+    radius_of_z(z) sampled at the increasing z_levels, rims at half-segment
+    angles (as tube_surface), fan caps at both ends. Outward normals from
+    each triangle's vertices."""
+    cx, cy = center
+    z = np.asarray(z_levels, dtype=np.float64)
+    th = 2.0 * np.pi * (np.arange(segments) + 0.5) / segments
+    r = np.asarray([radius_of_z(v) for v in z], dtype=np.float64)
+    P = np.stack([cx + r[:, None] * np.cos(th)[None, :], cy + r[:, None] * np.sin(th)[None, :],
+                  np.repeat(z[:, None], segments, axis=1)], axis=-1)  # (levels, segments, 3)
+    tris = []
+    for i in range(len(z) - 1):
+        for k in range(segments):
+            a, b = P[i, k], P[i, (k + 1) % segments]
+            c, d = P[i + 1, (k + 1) % segments], P[i + 1, k]
+            tris += [[a, b, c], [a, c, d]]
+    bot = np.array([cx, cy, z[0]])
+    top = np.array([cx, cy, z[-1]])
+    for k in range(segments):
+        tris.append([bot, P[0, (k + 1) % segments], P[0, k]])
+        tris.append([top, P[-1, k], P[-1, (k + 1) % segments]])
+    tris = np.array(tris)
+    nrm = _unit(np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0]))
+    return tris, nrm
+
+
+def stenosis_radius(radius=64.0, z_center=384.0, length=128.0, area_reduction=0.5):
+    """Cosine stenosis: r(z) = R (1 - d (1 + cos(2 pi (z - zc) / L)) / 2) for
+    |z - zc| <= L/2, R elsewhere, with d = 1 - sqrt(1 - area_reduction) so the
+    throat area is (1 - area_reduction) of the tube's."""
+    d = 1.0 - math.sqrt(1.0 - area_reduction)
+
+    def r(z):
+        if abs(z - z_center) <= 0.5 * length:
+            return radius * (1.0 - 0.5 * d * (1.0 + math.cos(2.0 * math.pi * (z - z_center) / length)))
+        return radius
+
+    return r
+
+
+def c2_stenosis(workdir, radius=64.0, length=1024.0, z_center=384.0, sten_length=128.0, area_reduction=0.5,
+                segments=256, tau=0.55, max_iter=20000):
+    """Stenosed tube: radius 64, length 1024, cosine stenosis of 50% area
+    reduction centred at z = 384 over 128 lu (a 130 x 130 x 1025 grid at
+    pitch 1). Inlet cap at z = 0 (normal -z), outlet cap at z = length
+    (normal +z). Returns (config path, info)."""
+    os.makedirs(workdir, exist_ok=True)
+    r = stenosis_radius(radius, z_center, sten_length, area_reduction)
+    z0, z1 = z_center - 0.5 * sten_length, z_center + 0.5 * sten_length
+    levels = sorted(set([0.0, float(length)] + list(np.arange(0.0, z0, 16.0)) + list(np.arange(z0, z1 + 1e-9, 2.0)) +
+                        list(np.arange(z1, length, 16.0))))
+    tris, nrm = revolution_surface(r, levels, segments=segments)
+    stl = os.path.join(workdir, "stenosis.stl")
+    write_stl(stl, tris, nrm, "stenosis")
+    cfg = os.path.join(workdir, "config.txt")
+    write_config(cfg, "stenosis.stl", tau=tau, max_iter=max_iter,
+                 inlets=[dict(plane="z", coordinate=0.0, normal=(0, 0, -1))],
+                 outlets=[dict(plane="z", coordinate=float(length), normal=(0, 0, 1))])
+    return cfg, dict(radius=radius, axis=(0.37, 0.21), n_triangles=len(tris), throat_radius=r(z_center))
+
+
 def tube_mask(radius, length, pad=1, center_offset=(0.37, 0.21)):
     """Fluid labels of a z-periodic tube sampled on the lattice directly (for
     the Poiseuille check): point (i, j) is fluid when its distance to the axis

@@ -146,7 +146,7 @@ struct AdjKern {
     const C base = phr - pu_u * ir;
 #pragma unroll
     for (int q = 0; q < Q; ++q)
-      lam[q] = (C(1) - om) * lam[q] + base + (pux * C(L::cx[q]) + puy * C(L::cy[q]) + puz * C(L::cz[q])) * irk;
+      lam[q] = (C(1) - om) * lam[q] + base + cdot<L, C>(q, pux, puy, puz) * irk;
     return double(-pu_u / (C(2) * k) - kap * rho * gu);
   }
 
@@ -212,7 +212,7 @@ struct AdjKern {
 #pragma unroll
     for (int q = 0; q < Q; ++q)
       a.own[int64_t(i) * Q + q] =
-          double(psi + (gx * C(L::cx[q]) + gy * C(L::cy[q]) + gz * C(L::cz[q])) / kp - gu);
+          double(psi + cdot<L, C>(q, gx, gy, gz) / kp - gu);
     return double(e.rho * gu / (C(2) * kp));
   }
 
@@ -226,7 +226,7 @@ struct AdjKern {
   __device__ static void pull_main(const AdjArgs& a, int32_t p, bool irr, int32_t row, int32_t seg, uint32_t bm,
                                    C* G) {
     if (irr) K::template pull_irregular<true>(a.c, p, row, G);
-    else K::template pull_regular<true>(a.c, p, seg, bm, G);
+    else K::template pull_regular<true>(a.c, p, K::load_deltas(a.c, seg), bm, G);
   }
 
   template <bool NTIN, bool NTOUT>
@@ -402,7 +402,7 @@ struct AdjKern {
       const C gu = gx * ux + gy * uy + gz * uz;
 #pragma unroll
       for (int q = 0; q < Q; ++q)
-        acc[q] += mu[q] + cst + fac * (gx * C(L::cx[q]) + gy * C(L::cy[q]) + gz * C(L::cz[q]) - gu);
+        acc[q] += mu[q] + cst + fac * (cdot<L, C>(q, gx, gy, gz) - gu);
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {

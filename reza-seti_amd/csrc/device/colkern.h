@@ -17,6 +17,13 @@ namespace colk {
 
 using namespace dev;
 
+// Four values of one near-wall node, loaded or stored by one instruction
+// (two for FP64).
+template <class C>
+struct alignas(4 * sizeof(C)) Quad {
+  C x, y, z, w;
+};
+
 template <class L, class S>
 struct ColKern {
   using P = Pop<S>;
@@ -26,11 +33,16 @@ struct ColKern {
 
   // Pull of a regular slot p in segment `seg`; bit q of `bm` marks a missing
   // link (half-way bounce-back: the node's own population opp(q)).
-  template <bool NTL = false>
-  __device__ static void pull_regular(const ColArgs& a, int32_t p, int32_t seg, uint32_t bm, C* F) {
+  struct Deltas {
+    int4 lo, hi;
+  };
+  __device__ static Deltas load_deltas(const ColArgs& a, int32_t seg) {
     const int4* d4 = reinterpret_cast<const int4*>(a.seg_delta) + 2 * int64_t(seg);
-    const int4 d0 = d4[0], d1 = d4[1];
-    const int32_t d[NCOL] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+    return Deltas{d4[0], d4[1]};
+  }
+  template <bool NTL = false>
+  __device__ static void pull_regular(const ColArgs& a, int32_t p, const Deltas& dd, uint32_t bm, C* F) {
+    const int32_t d[NCOL] = {dd.lo.x, dd.lo.y, dd.lo.z, dd.lo.w, dd.hi.x, dd.hi.y, dd.hi.z, dd.hi.w};
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int o = col_of(L::cx[q], L::cy[q]);
@@ -65,7 +77,7 @@ struct ColKern {
       pull_irregular(a, p, m.irr_base + rank_below(m.irr_mask, l), F);
     } else {
       const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
-      pull_regular(a, p, segment_of(m, l), bm, F);
+      pull_regular(a, p, load_deltas(a, segment_of(m, l)), bm, F);
     }
   }
 
@@ -91,14 +103,7 @@ struct ColKern {
     const C frc[3] = {C(a.force[0]), C(a.force[1]), C(a.force[2])};
     const C alpha = (FORCE && a.porosity) ? C(a.porosity[p]) : C(0);
     collide_ext<L, C, SHIFT, FORCE>(F, C(a.omega), frc, alpha, false, drho, ux, uy, uz);
-    if (i >= 0) {
-      C* e = static_cast<C*>(a.nw_rho_u);
-      const int64_t m = a.n_nw;
-      e[i] = drho;
-      e[m + i] = ux;
-      e[2 * m + i] = uy;
-      e[3 * m + i] = uz;
-    }
+    if (i >= 0) static_cast<Quad<C>*>(a.nw_rho_u)[i] = Quad<C>{drho, ux, uy, uz};
     // NT: nontemporal stores (a compile-time choice: a runtime branch
     // between two otherwise identical stores is merged and loses the hint).
 #pragma unroll
@@ -113,17 +118,25 @@ struct ColKern {
   //   tau_i = C sum_a (f_a - feq_a) (c_a.n) (c_ai - (c_a.n) n_i)
   // (Simulation.cpp:775-797 with its inner sums written out). feq is the
   // equilibrium of the collision that produced this step's input.
+  // With Pi = sum_a (f_a - feq_a) c_a c_a^T this is tau = C (Pi n - (n.Pi n) n).
+  // The fourth-order weight sums of D3Q19 and D3Q27 are isotropic, so the
+  // second-order equilibrium has the exact second moment rho/3 I + rho u u^T
+  // (shifted, w (drho + rho P): drho/3 I + rho u u^T), and Pi takes 42
+  // additions over F instead of a rebuilt f_eq per direction.
   struct Eq {
     C s, rho, ux, uy, uz;
   };
 
-  // That equilibrium's density and velocity, as stored by collide_store
-  // (four independent loads, issued with the pull).
-  __device__ static Eq stored_equilibrium(const ColArgs& a, int32_t i) {
-    const C* e = static_cast<const C*>(a.nw_rho_u);
-    const int64_t m = a.n_nw;
-    const C s = e[i];
-    return Eq{s, SHIFT ? C(1) + s : s, e[m + i], e[2 * m + i], e[3 * m + i]};
+  // Inputs of the stress of near-wall node i: the density (deviation) and
+  // velocity that collide_store kept, and the wall normal, both in the
+  // arithmetic precision (two vector loads).
+  struct NwIn {
+    C s, ux, uy, uz, nx, ny, nz;
+  };
+  __device__ static NwIn load_near_wall(const ColArgs& a, int32_t i) {
+    const Quad<C> e = static_cast<const Quad<C>*>(a.nw_rho_u)[i];
+    const Quad<C> n = static_cast<const Quad<C>*>(a.nw_normal_c)[i];
+    return NwIn{e.x, e.y, e.z, e.w, n.x, n.y, n.z};
   }
 
   // The same density and velocity recovered from the node's own
@@ -159,29 +172,40 @@ struct ColKern {
   // The stress of this step goes to ring slot a.ring_slot (4 values in the
   // arithmetic precision, plain stores, no dependent loads);
   // launch_wss_flush folds the ring into the FP64 accumulators.
-  __device__ static void wss_node(const ColArgs& a, int32_t i, const Eq& e, const C* F, double wt) {
+  __device__ static void wss_node(const ColArgs& a, int32_t i, const NwIn& e, const C* F, double wt) {
 #pragma clang fp contract(fast)
-    const C ux = e.ux, uy = e.uy, uz = e.uz, rho = e.rho, s = e.s;
-    const C usq = ux * ux + uy * uy + uz * uz;
-    const int64_t m = a.n_nw;
-    const C nx = C(a.nw_normal[i]), ny = C(a.nw_normal[m + i]), nz = C(a.nw_normal[2 * m + i]);
-    C tx = C(0), ty = C(0), tz = C(0);
+    Sum<C> Pxx, Pyy, Pzz, Pxy, Pxz, Pyz;
 #pragma unroll
-    for (int al = 0; al < Q; ++al) {
-      const C fne = F[al] - geq_of<L, C, SHIFT>(al, SHIFT ? s : C(0), rho, ux, uy, uz, usq);
-      const C cn = C(L::cx[al]) * nx + C(L::cy[al]) * ny + C(L::cz[al]) * nz;
-      tx += fne * cn * (C(L::cx[al]) - cn * nx);
-      ty += fne * cn * (C(L::cy[al]) - cn * ny);
-      tz += fne * cn * (C(L::cz[al]) - cn * nz);
+    for (int q = 0; q < Q; ++q) {
+      const int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      Pxx.acc(cx * cx, F[q]);
+      Pyy.acc(cy * cy, F[q]);
+      Pzz.acc(cz * cz, F[q]);
+      Pxy.acc(cx * cy, F[q]);
+      Pxz.acc(cx * cz, F[q]);
+      Pyz.acc(cy * cz, F[q]);
     }
+    C xx = Pxx.v, yy = Pyy.v, zz = Pzz.v, xy = Pxy.v, xz = Pxz.v, yz = Pyz.v;
+    const C ux = e.ux, uy = e.uy, uz = e.uz, rho = SHIFT ? C(1) + e.s : e.s;
+    const C iso = (SHIFT ? e.s : rho) * C(1.0 / 3.0);
+    const C rx = rho * ux, ry = rho * uy;
+    xx -= iso + rx * ux;
+    yy -= iso + ry * uy;
+    zz -= iso + rho * uz * uz;
+    xy -= rx * uy;
+    xz -= rx * uz;
+    yz -= ry * uz;
+    const int64_t m = a.n_nw;
+    const C nx = e.nx, ny = e.ny, nz = e.nz;
+    const C px = xx * nx + xy * ny + xz * nz;
+    const C py = xy * nx + yy * ny + yz * nz;
+    const C pz = xz * nx + yz * ny + zz * nz;
+    const C npn = px * nx + py * ny + pz * nz;
+    const C tx = px - npn * nx, ty = py - npn * ny, tz = pz - npn * nz;
     const C wc = C(a.wss_C);
     const C t0 = wc * tx, t1 = wc * ty, t2 = wc * tz;
     const C W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
-    C* r = static_cast<C*>(a.nw_ring) + int64_t(a.ring_slot) * 4 * m;
-    r[i] = t0;
-    r[m + i] = t1;
-    r[2 * m + i] = t2;
-    r[3 * m + i] = W;
+    static_cast<Quad<C>*>(a.nw_ring)[int64_t(a.ring_slot) * m + i] = Quad<C>{t0, t1, t2, W};
     // Objective: weighted sum of |tau|^2 over the chosen wall segment (wt
     // was loaded with the pull; zero outside the segment or the adjoint).
     if (wt != 0.0) a.obj_acc[i] += wt * double(W) * double(W);
@@ -192,6 +216,15 @@ struct ColKern {
   }
 
   // Slot p = lane l of a chunk whose metadata m is wave-uniform.
+  //
+  // Load order matters: the wait counter retires loads in issue order, and
+  // after a branch that issued loads the compiler waits for everything
+  // issued before the value it needs. So the stress inputs (raw loads, no
+  // arithmetic on them until the stress) go first, then the pull-address
+  // inputs (segment deltas, bounce-back mask; lanes without a mask read
+  // entry 0, the array holds at least one): waiting for the latter costs
+  // nothing more, and the pull is issued one memory latency after the
+  // chunk metadata.
   template <int MODE, bool FORCE, bool NT, bool NTL>
   __device__ static void main_slot(const ColArgs& a, int32_t p, int l, const ChunkMeta& m) {
     C F[Q];
@@ -200,21 +233,20 @@ struct ColKern {
     if (MODE != KMODE_INIT && irr && is_boundary_kind(a, a.irr_kind[row] & ROW_KIND_MASK)) return;
     const int32_t i = ((m.nw_mask >> l) & 1u) ? m.nw_base + rank_below(m.nw_mask, l) : -1;
     const bool nw = MODE == KMODE_STEP && a.wss_on && i >= 0;
-    Eq e{};
-    double wt = 0.0;
-    if (nw) {
-      e = stored_equilibrium(a, i);
-      if (a.obj_acc) wt = a.obj_w[i];
-    }
+    NwIn e{};
+    if (nw) e = load_near_wall(a, i);
+    __builtin_amdgcn_sched_barrier(0);
+    const Deltas dd = load_deltas(a, segment_of(m, l));
+    const bool bbl = (m.bb_mask >> l) & 1u;
+    const uint32_t bm_raw = a.bbm[bbl ? m.bb_base + rank_below(m.bb_mask, l) : 0];
     if (MODE == KMODE_INIT) {
       load_direct(a, p, F);
     } else if (irr) {
       pull_irregular<NTL>(a, p, row, F);
     } else {
-      const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
-      pull_regular<NTL>(a, p, segment_of(m, l), bm, F);
+      pull_regular<NTL>(a, p, dd, bbl ? bm_raw : 0u, F);
     }
-    if (nw) wss_node(a, i, e, F, wt);
+    if (nw) wss_node(a, i, e, F, a.obj_acc ? a.obj_w[i] : 0.0);
     if (MODE == KMODE_MATERIALIZE) {
       write_direct(a, p, F);
       return;
@@ -265,10 +297,10 @@ struct ColKern {
     const int32_t u = a.irr_bcn[row];
     const int32_t i = ((m.nw_mask >> l) & 1u) ? m.nw_base + rank_below(m.nw_mask, l) : -1;
     const bool nw = MODE == KMODE_STEP && a.wss_on && i >= 0;
-    Eq e{};
+    NwIn e{};
     double wt = 0.0;
     if (nw) {
-      e = stored_equilibrium(a, i);
+      e = load_near_wall(a, i);
       if (a.obj_acc) wt = a.obj_w[i];
     }
     C F[Q];

@@ -121,33 +121,74 @@ __device__ inline void collide_ref(double* f, bool wall, double omega, double& r
 //   u = (j + F/2) / (rho (1 + alpha/2)),  F_tot = F - alpha rho u,
 //   f* = f + omega (feq - f) + (1 - omega/2) w [3 (c - u).F_tot + 9 (c.u)(c.F_tot)].
 // SHIFT: populations are f - w, and drho = sum(f - w) = rho - 1.
+// c_q . (x, y, z) over the nonzero components of c_q only (q is a
+// compile-time constant once the loops are unrolled). Written as
+// x * cx + y * cy + z * cz, the zero terms would stay in the code: 0 * x is
+// not foldable when x may be infinite or NaN.
+template <class L, class C>
+__device__ inline C cdot(int q, C x, C y, C z) {
+  C r = C(0);
+  bool any = false;
+  auto add = [&](int c, C v) {
+    if (c == 0) return;
+    const C t = c > 0 ? v : -v;
+    r = any ? r + t : t;
+    any = true;
+  };
+  add(L::cx[q], x);
+  add(L::cy[q], y);
+  add(L::cz[q], z);
+  return r;
+}
+
+// w_q (drho + rho P_q) (SHIFT) or w_q rho (1 + P_q), with
+// P_q = 3 cu + 4.5 cu^2 - 1.5 u^2 evaluated as cu (3 + 4.5 cu) - 1.5 u^2.
 template <class L, class C, bool SHIFT>
 __device__ inline C geq_of(int q, C drho, C rho, C ux, C uy, C uz, C usq) {
 #pragma clang fp contract(fast)
-  const C cu = ux * C(L::cx[q]) + uy * C(L::cy[q]) + uz * C(L::cz[q]);
+  const C cu = cdot<L, C>(q, ux, uy, uz);
   const C wq = C(L::w[q]);
-  const C poly = C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq;
+  const C poly = cu * (C(3) + C(4.5) * cu) - C(1.5) * usq;
   if (SHIFT) return wq * (drho + rho * poly);
   return wq * rho * (C(1) + poly);
 }
 
+// A sum of terms selected at compile time that starts from its first term
+// (0 + x is not foldable: it differs from x for x = -0).
+template <class C>
+struct Sum {
+  C v = C(0);
+  bool any = false;
+  __device__ void add(C x) {
+    v = any ? v + x : x;
+    any = true;
+  }
+  __device__ void sub(C x) {
+    v = any ? v - x : -x;
+    any = true;
+  }
+  // += sign * x for a compile-time sign in {-1, 0, 1}
+  __device__ void acc(int sign, C x) {
+    if (sign > 0) add(x);
+    if (sign < 0) sub(x);
+  }
+};
+
 template <class L, class C>
 __device__ inline void moments(const C* f, C& s, C& jx, C& jy, C& jz) {
 #pragma clang fp contract(fast)
-  s = C(0);
-  jx = C(0);
-  jy = C(0);
-  jz = C(0);
+  Sum<C> S, X, Y, Z;
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
-    s += f[q];
-    if (L::cx[q] > 0) jx += f[q];
-    if (L::cx[q] < 0) jx -= f[q];
-    if (L::cy[q] > 0) jy += f[q];
-    if (L::cy[q] < 0) jy -= f[q];
-    if (L::cz[q] > 0) jz += f[q];
-    if (L::cz[q] < 0) jz -= f[q];
+    S.add(f[q]);
+    X.acc(L::cx[q], f[q]);
+    Y.acc(L::cy[q], f[q]);
+    Z.acc(L::cz[q], f[q]);
   }
+  s = S.v;
+  jx = X.v;
+  jy = Y.v;
+  jz = Z.v;
 }
 
 template <class L, class C, bool SHIFT, bool FORCE>
@@ -182,9 +223,8 @@ __device__ inline void collide_ext(C* f, C omega, const C* force, C alpha, bool 
     const C e = geq_of<L, C, SHIFT>(q, s, rho, ux, uy, uz, usq);
     C out = f[q] + omega * (e - f[q]);
     if (FORCE) {
-      const C cx = C(L::cx[q]), cy = C(L::cy[q]), cz = C(L::cz[q]);
-      const C cu = cx * ux + cy * uy + cz * uz;
-      const C cF = cx * Fx + cy * Fy + cz * Fz;
+      const C cu = cdot<L, C>(q, ux, uy, uz);
+      const C cF = cdot<L, C>(q, Fx, Fy, Fz);
       const C uF = ux * Fx + uy * Fy + uz * Fz;
       out += kf * C(L::w[q]) * (C(3) * (cF - uF) + C(9) * cu * cF);
     }

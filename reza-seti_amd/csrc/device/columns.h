@@ -94,10 +94,11 @@ struct ColArgs {
   const int32_t* irr_inlet;  // n_irr: inlet basis index, or -1
   int32_t n_nw;
   const double* nw_normal;   // 3 * n_nw (SoA), near-wall slot order
+  const void* nw_normal_c;   // n_nw x (nx, ny, nz, 0) in the arithmetic precision
   double* nw_wss;            // 5 * n_nw: |tau|, acc tau x/y/z, acc |tau| (FP64)
-  void* nw_ring;             // ring of per-step tau x/y/z, |tau| (arithmetic precision, 4 * n_nw per slot)
+  void* nw_ring;             // ring slots of n_nw x (tau x, y, z, |tau|), arithmetic precision
   int32_t ring_slot;         // slot this step writes
-  void* nw_rho_u;            // 4 * n_nw, arithmetic precision: density (deviation), u of the last collision
+  void* nw_rho_u;            // n_nw x (density (deviation), ux, uy, uz) of the last collision, arithmetic precision
   double wss_C;
   int32_t wss_on;
   const int32_t* bslots;     // inlet/outlet slots

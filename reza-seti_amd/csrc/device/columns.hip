@@ -70,12 +70,17 @@ __global__ __launch_bounds__(BLOCK) void wss_flush_kernel(ColArgs a, int32_t cou
   const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= a.n_nw) return;
   const int64_t m = a.n_nw;
-  const C* r = static_cast<const C*>(a.nw_ring);
+  const colk::Quad<C>* r = static_cast<const colk::Quad<C>*>(a.nw_ring);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int32_t k = 0; k < count; ++k)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[c] += double(r[(int64_t(k) * 4 + c) * m + i]);
-  a.nw_wss[i] = double(r[(int64_t(count - 1) * 4 + 3) * m + i]);
+  colk::Quad<C> v{};
+  for (int32_t k = 0; k < count; ++k) {
+    v = r[int64_t(k) * m + i];
+    acc[0] += double(v.x);
+    acc[1] += double(v.y);
+    acc[2] += double(v.z);
+    acc[3] += double(v.w);
+  }
+  a.nw_wss[i] = double(v.w);
 #pragma unroll
   for (int c = 0; c < 4; ++c) a.nw_wss[(1 + c) * m + i] += acc[c];
 }

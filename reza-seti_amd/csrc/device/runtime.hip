@@ -118,6 +118,7 @@ struct lbm_solver {
   int32_t* irr_inlet = nullptr;
   int32_t* bslots = nullptr;
   double* nw_normal = nullptr;
+  void* nw_normal_c = nullptr;    // (nx, ny, nz, 0) per near-wall node, arithmetic precision
   double* nw_wss = nullptr;
   void* nw_rho_u = nullptr;
   void* nw_ring = nullptr;
@@ -200,7 +201,7 @@ void free_all(lbm_solver* s) {
   void* ptrs[] = {s->fs[0], s->fs[1], s->fprev[0], s->fprev[1], s->src, s->sflags, s->bc_nbr, s->inlet_slot,
                   s->inlet_basis, s->normal, s->rho_u, s->wss, s->porosity, s->err, s->f_tmp, s->partials,
                   s->red_out, s->meta, s->seg_delta, s->bbm, s->irr_src, s->irr_kind, s->irr_bcn,
-                  s->irr_inlet, s->bslots, s->nw_normal, s->nw_wss, s->nw_rho_u, s->nw_ring};
+                  s->irr_inlet, s->bslots, s->nw_normal, s->nw_normal_c, s->nw_wss, s->nw_rho_u, s->nw_ring};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -533,6 +534,7 @@ int create_columns(lbm_solver* s, const Tables& t) {
       (rc = dalloc(s, &s->irr_bcn, ibcn.size())) || (rc = dalloc(s, &s->irr_inlet, iinl.size())) ||
       (rc = dalloc(s, &s->bslots, std::max<size_t>(bsl.size(), 1))) ||
       (rc = dalloc(s, &s->nw_normal, 3 * nnw1)) || (rc = dalloc(s, &s->nw_wss, 5 * nnw1)) ||
+      (rc = dalloc_raw(s, &s->nw_normal_c, 4 * nnw1 * (s->prec == 0 ? 8 : 4))) ||
       (rc = dalloc_raw(s, &s->nw_rho_u, 4 * nnw1 * (s->prec == 0 ? 8 : 4))) ||
       (rc = dalloc_raw(s, &s->nw_ring, size_t(s->ring_depth) * 4 * nnw1 * (s->prec == 0 ? 8 : 4))) ||
       (rc = dalloc(s, &s->err, 1)) || (rc = dalloc(s, &s->inlet_basis, MAX_MODES * ninl)) ||
@@ -541,6 +543,11 @@ int create_columns(lbm_solver* s, const Tables& t) {
     return rc;
   std::vector<double> basis(MAX_MODES * ninl, 0.0);
   for (size_t i = 0; i < ninl; ++i) basis[i] = 1.0;
+  // AoS copies of the normals in the arithmetic precision.
+  std::vector<double> nrm4(4 * nnw1, 0.0);
+  for (int64_t i = 0; i < nnw; ++i)
+    for (int d = 0; d < 3; ++d) nrm4[4 * i + d] = nrm[static_cast<size_t>(d) * nnw + i];
+  const std::vector<float> nrm4_f(nrm4.begin(), nrm4.end());
   auto up = [&](void* d, const void* h, size_t bytes) {
     return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s->stream) : hipSuccess;
   };
@@ -549,6 +556,8 @@ int create_columns(lbm_solver* s, const Tables& t) {
       up(s->irr_src, isrc.data(), isrc.size() * 4) || up(s->irr_kind, ikind.data(), ikind.size() * 4) ||
       up(s->irr_bcn, ibcn.data(), ibcn.size() * 4) || up(s->irr_inlet, iinl.data(), iinl.size() * 4) ||
       up(s->bslots, bsl.data(), bsl.size() * 4) || up(s->nw_normal, nrm.data(), nrm.size() * 8) ||
+      (s->prec == 0 ? up(s->nw_normal_c, nrm4.data(), nrm4.size() * 8)
+                    : up(s->nw_normal_c, nrm4_f.data(), nrm4_f.size() * 4)) ||
       up(s->inlet_basis, basis.data(), basis.size() * 8) ||
       hipMemsetAsync(s->nw_wss, 0, 5 * nnw1 * 8, s->stream) ||
       hipMemsetAsync(s->nw_rho_u, 0, 4 * nnw1 * (s->prec == 0 ? 8 : 4), s->stream) ||
@@ -579,6 +588,7 @@ int create_columns(lbm_solver* s, const Tables& t) {
   c.irr_inlet = s->irr_inlet;
   c.n_nw = static_cast<int32_t>(nnw);
   c.nw_normal = s->nw_normal;
+  c.nw_normal_c = s->nw_normal_c;
   c.nw_wss = s->nw_wss;
   c.nw_rho_u = s->nw_rho_u;
   c.nw_ring = s->nw_ring;

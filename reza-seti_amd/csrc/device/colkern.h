@@ -24,6 +24,25 @@ struct alignas(4 * sizeof(C)) Quad {
   C x, y, z, w;
 };
 
+// Nontemporal store and load of a Quad (streamed data: written once, read
+// once much later).
+template <class C>
+__device__ inline void store_quad_nt(Quad<C>* dst, const Quad<C>& v) {
+  typedef C V4 __attribute__((ext_vector_type(4)));
+  V4 x;
+  x.x = v.x;
+  x.y = v.y;
+  x.z = v.z;
+  x.w = v.w;
+  __builtin_nontemporal_store(x, reinterpret_cast<V4*>(dst));
+}
+template <class C>
+__device__ inline Quad<C> load_quad_nt(const Quad<C>* src) {
+  typedef C V4 __attribute__((ext_vector_type(4)));
+  const V4 x = __builtin_nontemporal_load(reinterpret_cast<const V4*>(src));
+  return Quad<C>{x.x, x.y, x.z, x.w};
+}
+
 template <class L, class S>
 struct ColKern {
   using P = Pop<S>;
@@ -40,25 +59,82 @@ struct ColKern {
     const int4* d4 = reinterpret_cast<const int4*>(a.seg_delta) + 2 * int64_t(seg);
     return Deltas{d4[0], d4[1]};
   }
-  template <bool NTL = false>
-  __device__ static void pull_regular(const ColArgs& a, int32_t p, const Deltas& dd, uint32_t bm, C* F) {
+  //
+  // A32: the state buffer has fewer than 2^32 bytes, so every access is the
+  // buffer's scalar base plus a 32-bit byte offset (q npad + slot) size:
+  // the plane term q npad size is wave-uniform, the slot term one add per
+  // lateral offset, and a bounce-back link swaps in the own-slot offset of
+  // plane opp(q) with one bit-select. Otherwise 64-bit element indices.
+  // Byte offsets of the regular pull of slot p (A32 layout, see below).
+  __device__ static void regular_offsets(const ColArgs& a, int32_t p, const Deltas& dd, uint32_t bm, uint32_t* off) {
+    constexpr uint32_t SZ = sizeof(S);
     const int32_t d[NCOL] = {dd.lo.x, dd.lo.y, dd.lo.z, dd.lo.w, dd.hi.x, dd.hi.y, dd.hi.z, dd.hi.w};
+    const uint32_t plane = uint32_t(a.npad) * SZ;
+    const uint32_t own = uint32_t(p) * SZ;
+    uint32_t t[NCOL];
+#pragma unroll
+    for (int o = 0; o < NCOL; ++o) t[o] = own + uint32_t(d[o]) * SZ;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int o = col_of(L::cx[q], L::cy[q]);
-      const int32_t s = (o < 0 ? p : p + d[o]) - L::cz[q];
-      const bool bb = (bm >> q) & 1u;
-      const int64_t idx = int64_t(bb ? L::opp[q] : q) * a.npad + (bb ? p : s);
-      if constexpr (NTL) F[q] = P::ld_nt(a.fs_in, idx);
-      else F[q] = P::ld(a.fs_in, idx);
+      // (wrapping arithmetic: the sums are exact byte offsets < 2^32)
+      const uint32_t reg = (o < 0 ? own : t[o]) + (uint32_t(q) * plane - uint32_t(L::cz[q] * int32_t(SZ)));
+      const uint32_t bb = own + uint32_t(L::opp[q]) * plane;
+      const uint32_t mask = uint32_t(__builtin_amdgcn_sbfe(int32_t(bm), q, 1));  // 0 or ~0
+      off[q] = (bb & mask) | (reg & ~mask);
+    }
+  }
+
+  // Byte offsets of the pull of explicit row `row` (-1 = bounce back).
+  __device__ static void irregular_offsets(const ColArgs& a, int32_t p, int32_t row, uint32_t* off) {
+    constexpr uint32_t SZ = sizeof(S);
+    int32_t src[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) src[q] = a.irr_src[int64_t(q) * a.n_irr + row];
+    const uint32_t plane = uint32_t(a.npad) * SZ;
+    const uint32_t own = uint32_t(p) * SZ;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const uint32_t mask = uint32_t(src[q] >> 31);
+      const uint32_t reg = uint32_t(src[q]) * SZ + uint32_t(q) * plane;
+      const uint32_t bb = own + uint32_t(L::opp[q]) * plane;
+      off[q] = (bb & mask) | (reg & ~mask);
+    }
+  }
+
+  template <bool NTL = false, bool A32 = false>
+  __device__ static void pull_regular(const ColArgs& a, int32_t p, const Deltas& dd, uint32_t bm, C* F) {
+    const int32_t d[NCOL] = {dd.lo.x, dd.lo.y, dd.lo.z, dd.lo.w, dd.hi.x, dd.hi.y, dd.hi.z, dd.hi.w};
+    if constexpr (A32) {
+      uint32_t off[Q];
+      regular_offsets(a, p, dd, bm, off);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) F[q] = ld_off<S, NTL>(a.fs_in, off[q]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int o = col_of(L::cx[q], L::cy[q]);
+        const int32_t s = (o < 0 ? p : p + d[o]) - L::cz[q];
+        const bool bb = (bm >> q) & 1u;
+        const int64_t idx = int64_t(bb ? L::opp[q] : q) * a.npad + (bb ? p : s);
+        if constexpr (NTL) F[q] = P::ld_nt(a.fs_in, idx);
+        else F[q] = P::ld(a.fs_in, idx);
+      }
     }
   }
 
   // Pull of an explicit row.
   // (The same cache hint as the regular pull: the compiler merges the two
   // branches' loads and would otherwise drop the hint.)
-  template <bool NTL = false>
+  template <bool NTL = false, bool A32 = false>
   __device__ static void pull_irregular(const ColArgs& a, int32_t p, int32_t row, C* F) {
+    if constexpr (A32) {
+      uint32_t off[Q];
+      irregular_offsets(a, p, row, off);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) F[q] = ld_off<S, NTL>(a.fs_in, off[q]);
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int32_t s = a.irr_src[int64_t(q) * a.n_irr + row];
@@ -97,7 +173,7 @@ struct ColKern {
   // Collide and store; near-wall node i (>= 0) also keeps the density
   // (or its deviation) and velocity of this collision for the next step's
   // wall shear stress.
-  template <bool FORCE, bool NT>
+  template <bool FORCE, bool NT, bool A32 = false>
   __device__ static void collide_store(const ColArgs& a, int32_t p, C* F, int32_t i) {
     C drho, ux, uy, uz;
     const C frc[3] = {C(a.force[0]), C(a.force[1]), C(a.force[2])};
@@ -108,8 +184,14 @@ struct ColKern {
     // between two otherwise identical stores is merged and loses the hint).
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      if constexpr (NT) P::st_nt(a.fs_out, int64_t(q) * a.npad + p, F[q]);
-      else P::st(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+      if constexpr (A32) {
+        // plane base (scalar) + own-slot offset (one vector register for all q)
+        st_off<S, NT>(static_cast<char*>(a.fs_out) + q * a.npad * int64_t(sizeof(S)), uint32_t(p) * uint32_t(sizeof(S)),
+                      F[q]);
+      } else {
+        if constexpr (NT) P::st_nt(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+        else P::st(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+      }
     }
   }
 
@@ -205,7 +287,9 @@ struct ColKern {
     const C wc = C(a.wss_C);
     const C t0 = wc * tx, t1 = wc * ty, t2 = wc * tz;
     const C W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
-    static_cast<Quad<C>*>(a.nw_ring)[int64_t(a.ring_slot) * m + i] = Quad<C>{t0, t1, t2, W};
+    // Streamed past the caches: the ring is read back only by the flush,
+    // and must not displace the state a cache-resident step relies on.
+    store_quad_nt(static_cast<Quad<C>*>(a.nw_ring) + int64_t(a.ring_slot) * m + i, Quad<C>{t0, t1, t2, W});
     // Objective: weighted sum of |tau|^2 over the chosen wall segment (wt
     // was loaded with the pull; zero outside the segment or the adjoint).
     if (wt != 0.0) a.obj_acc[i] += wt * double(W) * double(W);
@@ -225,7 +309,7 @@ struct ColKern {
   // entry 0, the array holds at least one): waiting for the latter costs
   // nothing more, and the pull is issued one memory latency after the
   // chunk metadata.
-  template <int MODE, bool FORCE, bool NT, bool NTL>
+  template <int MODE, bool FORCE, bool NT, bool NTL, bool A32 = false>
   __device__ static void main_slot(const ColArgs& a, int32_t p, int l, const ChunkMeta& m) {
     C F[Q];
     const bool irr = (m.irr_mask >> l) & 1u;
@@ -241,6 +325,14 @@ struct ColKern {
     const uint32_t bm_raw = a.bbm[bbl ? m.bb_base + rank_below(m.bb_mask, l) : 0];
     if (MODE == KMODE_INIT) {
       load_direct(a, p, F);
+    } else if constexpr (A32) {
+      // Offsets from either branch, then one set of scalar-base loads (loads
+      // written in both branches would be merged with 64-bit addresses).
+      uint32_t off[Q];
+      if (irr) irregular_offsets(a, p, row, off);
+      else regular_offsets(a, p, dd, bbl ? bm_raw : 0u, off);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) F[q] = ld_off<S, NTL>(a.fs_in, off[q]);
     } else if (irr) {
       pull_irregular<NTL>(a, p, row, F);
     } else {
@@ -251,7 +343,7 @@ struct ColKern {
       write_direct(a, p, F);
       return;
     }
-    collide_store<FORCE, NT>(a, p, F, i);
+    collide_store<FORCE, NT, A32>(a, p, F, i);
   }
 
   // Unrolled to the fixed mode bound so the basis loads issue together.

@@ -6,6 +6,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../lattice.h"
 
 namespace lbm {
@@ -61,6 +63,36 @@ struct Pop<__half> {
     __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(v)), static_cast<unsigned short*>(base) + i);
   }
 };
+
+// Element at byte offset `off` of a wave-uniform base: the scalar base +
+// 32-bit vector offset addressing form, no 64-bit address arithmetic per
+// lane (buffers of fewer than 2^32 bytes).
+template <class S, bool NT>
+__device__ inline typename Pop<S>::C ld_off(const void* base, uint32_t off) {
+  const char* b = static_cast<const char*>(base) + off;
+  if constexpr (std::is_same<S, __half>::value) {
+    const unsigned short* q = reinterpret_cast<const unsigned short*>(b);
+    return __half2float(__ushort_as_half(NT ? __builtin_nontemporal_load(q) : *q));
+  } else {
+    const S* q = reinterpret_cast<const S*>(b);
+    if constexpr (NT) return __builtin_nontemporal_load(q);
+    else return *q;
+  }
+}
+template <class S, bool NT>
+__device__ inline void st_off(void* base, uint32_t off, typename Pop<S>::C v) {
+  char* b = static_cast<char*>(base) + off;
+  if constexpr (std::is_same<S, __half>::value) {
+    unsigned short* q = reinterpret_cast<unsigned short*>(b);
+    const unsigned short h = __half_as_ushort(__float2half_rn(v));
+    if constexpr (NT) __builtin_nontemporal_store(h, q);
+    else *q = h;
+  } else {
+    S* q = reinterpret_cast<S*>(b);
+    if constexpr (NT) __builtin_nontemporal_store(S(v), q);
+    else *q = S(v);
+  }
+}
 
 // ------------------------------------------------- reference FP64 collision
 // Velocity sums grouped exactly as Simulation.cpp:100-134 (D3Q19) and

@@ -25,7 +25,7 @@ namespace {
 using namespace dev;
 using colk::ColKern;
 
-template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32>
 __device__ inline void columns_body(const ColArgs& a) {
   const int32_t b = blockIdx.x;
   if (b < a.b_blocks) {
@@ -42,12 +42,12 @@ __device__ inline void columns_body(const ColArgs& a) {
   // One wavefront covers one chunk: its metadata is wave-uniform.
   const int32_t chunk = __builtin_amdgcn_readfirstlane(p / CHUNK);
   const ChunkMeta m = a.meta[chunk];
-  if (p < a.n) ColKern<L, S>::template main_slot<MODE, FORCE, NT, NTL>(a, p, p & (CHUNK - 1), m);
+  if (p < a.n) ColKern<L, S>::template main_slot<MODE, FORCE, NT, NTL, A32>(a, p, p & (CHUNK - 1), m);
 }
 
-template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32>
 __global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
-  columns_body<L, S, MODE, FORCE, NT, NTL>(a);
+  columns_body<L, S, MODE, FORCE, NT, NTL, A32>(a);
 }
 
 // Occupancy-bounded variant for the float-arithmetic precisions: the
@@ -58,10 +58,10 @@ __global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
 #define LBM_COLUMNS_WAVES_PER_EU 7
 #endif
 #if LBM_COLUMNS_WAVES_PER_EU > 0
-template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32>
 __global__ __attribute__((amdgpu_waves_per_eu(LBM_COLUMNS_WAVES_PER_EU))) __launch_bounds__(BLOCK) void
 columns_kernel_occ(ColArgs a) {
-  columns_body<L, S, MODE, FORCE, NT, NTL>(a);
+  columns_body<L, S, MODE, FORCE, NT, NTL, A32>(a);
 }
 #endif
 
@@ -69,20 +69,25 @@ template <class C>
 __global__ __launch_bounds__(BLOCK) void wss_flush_kernel(ColArgs a, int32_t count) {
   const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= a.n_nw) return;
+  // Everything here is touched once per flush: nontemporal accesses keep
+  // it from displacing a cache-resident state.
   const int64_t m = a.n_nw;
   const colk::Quad<C>* r = static_cast<const colk::Quad<C>*>(a.nw_ring);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   colk::Quad<C> v{};
   for (int32_t k = 0; k < count; ++k) {
-    v = r[int64_t(k) * m + i];
+    v = colk::load_quad_nt(r + int64_t(k) * m + i);
     acc[0] += double(v.x);
     acc[1] += double(v.y);
     acc[2] += double(v.z);
     acc[3] += double(v.w);
   }
-  a.nw_wss[i] = double(v.w);
+  __builtin_nontemporal_store(double(v.w), a.nw_wss + i);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) a.nw_wss[(1 + c) * m + i] += acc[c];
+  for (int c = 0; c < 4; ++c) {
+    double* t = a.nw_wss + (1 + c) * m + i;
+    __builtin_nontemporal_store(__builtin_nontemporal_load(t) + acc[c], t);
+  }
 }
 
 __global__ __launch_bounds__(BLOCK) void wss_stats_cols(ColArgs a, double* partials) {
@@ -116,27 +121,34 @@ __global__ __launch_bounds__(BLOCK) void wss_stats_cols(ColArgs a, double* parti
   }
 }
 
-template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32 = false>
 hipError_t launch_one(const ColArgs& a, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>(a.b_blocks + a.m_blocks);
   if (!grid) return hipSuccess;
 #if LBM_COLUMNS_WAVES_PER_EU > 0
   if constexpr (!std::is_same<S, double>::value) {
-    hipLaunchKernelGGL((columns_kernel_occ<L, S, MODE, FORCE, NT, NTL>), dim3(grid), dim3(BLOCK), 0, st, a);
+    hipLaunchKernelGGL((columns_kernel_occ<L, S, MODE, FORCE, NT, NTL, A32>), dim3(grid), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
   }
 #endif
-  hipLaunchKernelGGL((columns_kernel<L, S, MODE, FORCE, NT, NTL>), dim3(grid), dim3(BLOCK), 0, st, a);
+  hipLaunchKernelGGL((columns_kernel<L, S, MODE, FORCE, NT, NTL, A32>), dim3(grid), dim3(BLOCK), 0, st, a);
   return hipGetLastError();
 }
 
+template <class L, class S, bool FORCE, bool A32>
+hipError_t launch_step_hint(const ColArgs& a, hipStream_t st) {
+  switch (a.cache_hint) {
+    case CACHE_NT_STORES: return launch_one<L, S, KMODE_STEP, FORCE, true, false, A32>(a, st);
+    case CACHE_NT_LOADS: return launch_one<L, S, KMODE_STEP, FORCE, false, true, A32>(a, st);
+  }
+  return launch_one<L, S, KMODE_STEP, FORCE, false, false, A32>(a, st);
+}
+
+// 32-bit byte offsets when the whole state buffer spans fewer than 2^32 bytes.
 template <class L, class S, bool FORCE>
 hipError_t launch_step(const ColArgs& a, hipStream_t st) {
-  switch (a.cache_hint) {
-    case CACHE_NT_STORES: return launch_one<L, S, KMODE_STEP, FORCE, true, false>(a, st);
-    case CACHE_NT_LOADS: return launch_one<L, S, KMODE_STEP, FORCE, false, true>(a, st);
-  }
-  return launch_one<L, S, KMODE_STEP, FORCE, false, false>(a, st);
+  const bool a32 = uint64_t(L::Q) * uint64_t(a.npad) * sizeof(S) < (uint64_t(1) << 32);
+  return a32 ? launch_step_hint<L, S, FORCE, true>(a, st) : launch_step_hint<L, S, FORCE, false>(a, st);
 }
 
 template <class L, class S>

@@ -323,7 +323,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "C1 precision-study Y-bifurcation, D3Q19, FP32-shifted populations",
-                   "nodes": N, "special_nodes": int(sinfo.n_special), "triangles": info["n_triangles"],
+                   "nodes": N, "explicit_row_nodes": int(sinfo.n_special), "triangles": info["n_triangles"],
                    "inlet": f"Womersley Wo={WO} period {PERIOD} steps, U_mean {U_MEAN}",
                    "outlet": "NEE pressure rho=1", "walls": "half-way bounce-back", "tau": round(tau_for(), 6)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -269,6 +269,9 @@ int lbm_synchronize(lbm_solver* s);
 void lbm_destroy(lbm_solver* s);
 
 typedef struct {
+  /* n_special: nodes on the general path (reference wall rule: special
+   * slots; half-way bounce-back: nodes with an explicit pull row);
+   * n_bulk: the others. */
   int64_t n_nodes, n_special, n_bulk, n_pad;
   int64_t device_bytes;
   int32_t q, precision;

@@ -378,47 +378,54 @@ int create_columns(lbm_solver* s, const Tables& t) {
   };
   const bool inl_b = p.inlet_mode != LBM_INLET_NONE, out_b = p.outlet_mode != LBM_OUTLET_NONE;
 
-  // Column segments: maximal runs of slots with the same (i, j) and
-  // consecutive k.
-  std::vector<int32_t> seg_of(N), seg_start;
-  for (int64_t n = 0; n < N; ++n) {
-    const int32_t* c = &t.ijk[3 * static_cast<size_t>(n)];
-    if (n == 0 || c[0] != c[-3] || c[1] != c[-2] || c[2] != c[-1] + 1) seg_start.push_back(static_cast<int32_t>(n));
-    seg_of[n] = static_cast<int32_t>(seg_start.size() - 1);
-  }
-  const int64_t nseg = static_cast<int64_t>(seg_start.size());
-  seg_start.push_back(static_cast<int32_t>(N));
-
-  // Per segment and lateral offset o, the delta src + c_z - n that most of
-  // its links agree on (periodic wraps and a neighbour column that changes
-  // segment disagree; their nodes take explicit rows).
-  std::vector<int32_t> delta(static_cast<size_t>(NCOL) * std::max<int64_t>(nseg, 1), 0);
+  // Column segments: runs of slots with the same (i, j), consecutive k and,
+  // for every lateral offset o, one delta src + c_z - n that all existing
+  // links of the run's regular nodes share. Walking the slots in order, a
+  // node whose links disagree with the deltas its run has established so far
+  // (the neighbour column changed segment) starts a new segment; a node that
+  // disagrees with itself (a periodic wrap, a link of its own column that is
+  // not the next slot) or is an inlet/outlet node takes an explicit row and
+  // constrains nothing. Deltas no link establishes stay 0.
+  std::vector<int32_t> seg_of(N), delta;
   {
-    std::vector<int32_t> cand;
-    for (int64_t sg = 0; sg < nseg; ++sg)
-      for (int o = 0; o < NCOL; ++o) {
-        cand.clear();
-        for (int64_t n = seg_start[sg]; n < seg_start[sg + 1]; ++n)
-          for (int q = 0; q < Q; ++q) {
-            if (col_of(L.cx[q], L.cy[q]) != o) continue;
-            const int32_t sq = srcq(q, n);
-            if (sq >= 0) cand.push_back(static_cast<int32_t>(int64_t(sq) + L.cz[q] - n));
-          }
-        if (cand.empty()) continue;
-        std::sort(cand.begin(), cand.end());
-        int32_t best = cand[0];
-        size_t best_run = 0;
-        for (size_t i = 0; i < cand.size();) {
-          size_t j = i;
-          while (j < cand.size() && cand[j] == cand[i]) ++j;
-          if (j - i > best_run) {
-            best_run = j - i;
-            best = cand[i];
-          }
-          i = j;
+    int32_t cur[NCOL];
+    bool have[NCOL] = {};
+    for (int64_t n = 0; n < N; ++n) {
+      const int32_t* c = &t.ijk[3 * static_cast<size_t>(n)];
+      bool split = n == 0 || c[0] != c[-3] || c[1] != c[-2] || c[2] != c[-1] + 1;
+      bool self_ok = !((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b));
+      int32_t nd[NCOL];
+      bool nh[NCOL] = {};
+      for (int q = 0; q < Q && self_ok; ++q) {
+        const int32_t sq = srcq(q, n);
+        if (sq < 0) continue;
+        const int o = col_of(L.cx[q], L.cy[q]);
+        if (o < 0) {
+          if (int64_t(sq) != n - L.cz[q]) self_ok = false;
+          continue;
         }
-        delta[static_cast<size_t>(sg) * NCOL + o] = best;
+        const int32_t dl = static_cast<int32_t>(int64_t(sq) + L.cz[q] - n);
+        if (nh[o] && nd[o] != dl) self_ok = false;
+        nd[o] = dl;
+        nh[o] = true;
       }
+      if (self_ok && !split)
+        for (int o = 0; o < NCOL; ++o)
+          if (nh[o] && have[o] && cur[o] != nd[o]) split = true;
+      if (split) {
+        delta.insert(delta.end(), NCOL, 0);
+        std::fill(have, have + NCOL, false);
+      }
+      if (self_ok)
+        for (int o = 0; o < NCOL; ++o)
+          if (nh[o] && !have[o]) {
+            cur[o] = nd[o];
+            have[o] = true;
+            delta[delta.size() - NCOL + o] = nd[o];
+          }
+      seg_of[n] = static_cast<int32_t>(delta.size() / NCOL - 1);
+    }
+    if (delta.empty()) delta.assign(NCOL, 0);
   }
 
   // Regular nodes: every existing link follows its segment's delta (or the
@@ -1452,6 +1459,10 @@ int lbm_get_solver_info(const lbm_solver* s, lbm_solver_info* o) {
   o->n_nodes = s->N;
   o->n_special = s->n_special;
   o->n_bulk = s->n_bulk;
+  if (s->prm.wall_mode == LBM_WALL_HALFWAY_BB) {  // column segments: explicit rows vs segment-decoded slots
+    o->n_special = s->n_irr;
+    o->n_bulk = s->N - s->n_irr;
+  }
   o->n_pad = s->npad;
   o->device_bytes = static_cast<int64_t>(s->device_bytes);
   o->q = s->Q;

@@ -173,7 +173,9 @@ struct ColKern {
   // Collide and store; near-wall node i (>= 0) also keeps the density
   // (or its deviation) and velocity of this collision for the next step's
   // wall shear stress.
-  template <bool FORCE, bool NT, bool A32 = false>
+  // KRES: planes [0, KRES) take plain accesses whatever the step's hint
+  // (they stay cache-resident in both state buffers).
+  template <bool FORCE, bool NT, bool A32 = false, int KRES = 0>
   __device__ static void collide_store(const ColArgs& a, int32_t p, C* F, int32_t i) {
     C drho, ux, uy, uz;
     const C frc[3] = {C(a.force[0]), C(a.force[1]), C(a.force[2])};
@@ -186,8 +188,12 @@ struct ColKern {
     for (int q = 0; q < Q; ++q) {
       if constexpr (A32) {
         // plane base (scalar) + own-slot offset (one vector register for all q)
-        st_off<S, NT>(static_cast<char*>(a.fs_out) + q * a.npad * int64_t(sizeof(S)), uint32_t(p) * uint32_t(sizeof(S)),
-                      F[q]);
+        if (q < KRES)
+          st_off<S, false>(static_cast<char*>(a.fs_out) + q * a.npad * int64_t(sizeof(S)),
+                           uint32_t(p) * uint32_t(sizeof(S)), F[q]);
+        else
+          st_off<S, NT>(static_cast<char*>(a.fs_out) + q * a.npad * int64_t(sizeof(S)), uint32_t(p) * uint32_t(sizeof(S)),
+                        F[q]);
       } else {
         if constexpr (NT) P::st_nt(a.fs_out, int64_t(q) * a.npad + p, F[q]);
         else P::st(a.fs_out, int64_t(q) * a.npad + p, F[q]);
@@ -309,7 +315,7 @@ struct ColKern {
   // entry 0, the array holds at least one): waiting for the latter costs
   // nothing more, and the pull is issued one memory latency after the
   // chunk metadata.
-  template <int MODE, bool FORCE, bool NT, bool NTL, bool A32 = false>
+  template <int MODE, bool FORCE, bool NT, bool NTL, bool A32 = false, int KRES = 0>
   __device__ static void main_slot(const ColArgs& a, int32_t p, int l, const ChunkMeta& m) {
     C F[Q];
     const bool irr = (m.irr_mask >> l) & 1u;
@@ -332,7 +338,10 @@ struct ColKern {
       if (irr) irregular_offsets(a, p, row, off);
       else regular_offsets(a, p, dd, bbl ? bm_raw : 0u, off);
 #pragma unroll
-      for (int q = 0; q < Q; ++q) F[q] = ld_off<S, NTL>(a.fs_in, off[q]);
+      for (int q = 0; q < Q; ++q) {
+        if (q < KRES) F[q] = ld_off<S, false>(a.fs_in, off[q]);
+        else F[q] = ld_off<S, NTL>(a.fs_in, off[q]);
+      }
     } else if (irr) {
       pull_irregular<NTL>(a, p, row, F);
     } else {
@@ -343,7 +352,7 @@ struct ColKern {
       write_direct(a, p, F);
       return;
     }
-    collide_store<FORCE, NT, A32>(a, p, F, i);
+    collide_store<FORCE, NT, A32, KRES>(a, p, F, i);
   }
 
   // Unrolled to the fixed mode bound so the basis loads issue together.

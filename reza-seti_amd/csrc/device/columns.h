@@ -114,6 +114,7 @@ struct ColArgs {
   double omega;
   double force[3];
   int32_t cache_hint;     // CACHE_PLAIN, CACHE_NT_STORES or CACHE_NT_LOADS
+  int32_t resident_planes;  // 0, 1 or 2: planes [0, k) of both state buffers stay cached (plain accesses)
   const float* porosity;  // per slot, or null
   const double* f_direct; // init input (FP64 unshifted, q * npad + slot), null = w_q
   double* f_out;          // materialize output

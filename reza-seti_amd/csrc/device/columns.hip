@@ -25,7 +25,7 @@ namespace {
 using namespace dev;
 using colk::ColKern;
 
-template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32, int KRES>
 __device__ inline void columns_body(const ColArgs& a) {
   const int32_t b = blockIdx.x;
   if (b < a.b_blocks) {
@@ -42,12 +42,12 @@ __device__ inline void columns_body(const ColArgs& a) {
   // One wavefront covers one chunk: its metadata is wave-uniform.
   const int32_t chunk = __builtin_amdgcn_readfirstlane(p / CHUNK);
   const ChunkMeta m = a.meta[chunk];
-  if (p < a.n) ColKern<L, S>::template main_slot<MODE, FORCE, NT, NTL, A32>(a, p, p & (CHUNK - 1), m);
+  if (p < a.n) ColKern<L, S>::template main_slot<MODE, FORCE, NT, NTL, A32, KRES>(a, p, p & (CHUNK - 1), m);
 }
 
-template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32, int KRES>
 __global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
-  columns_body<L, S, MODE, FORCE, NT, NTL, A32>(a);
+  columns_body<L, S, MODE, FORCE, NT, NTL, A32, KRES>(a);
 }
 
 // Occupancy-bounded variant for the float-arithmetic precisions: the
@@ -58,10 +58,10 @@ __global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
 #define LBM_COLUMNS_WAVES_PER_EU 7
 #endif
 #if LBM_COLUMNS_WAVES_PER_EU > 0
-template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32, int KRES>
 __global__ __attribute__((amdgpu_waves_per_eu(LBM_COLUMNS_WAVES_PER_EU))) __launch_bounds__(BLOCK) void
 columns_kernel_occ(ColArgs a) {
-  columns_body<L, S, MODE, FORCE, NT, NTL, A32>(a);
+  columns_body<L, S, MODE, FORCE, NT, NTL, A32, KRES>(a);
 }
 #endif
 
@@ -121,25 +121,35 @@ __global__ __launch_bounds__(BLOCK) void wss_stats_cols(ColArgs a, double* parti
   }
 }
 
-template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32 = false>
+template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32 = false, int KRES = 0>
 hipError_t launch_one(const ColArgs& a, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>(a.b_blocks + a.m_blocks);
   if (!grid) return hipSuccess;
 #if LBM_COLUMNS_WAVES_PER_EU > 0
   if constexpr (!std::is_same<S, double>::value) {
-    hipLaunchKernelGGL((columns_kernel_occ<L, S, MODE, FORCE, NT, NTL, A32>), dim3(grid), dim3(BLOCK), 0, st, a);
+    hipLaunchKernelGGL((columns_kernel_occ<L, S, MODE, FORCE, NT, NTL, A32, KRES>), dim3(grid), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
   }
 #endif
-  hipLaunchKernelGGL((columns_kernel<L, S, MODE, FORCE, NT, NTL, A32>), dim3(grid), dim3(BLOCK), 0, st, a);
+  hipLaunchKernelGGL((columns_kernel<L, S, MODE, FORCE, NT, NTL, A32, KRES>), dim3(grid), dim3(BLOCK), 0, st, a);
   return hipGetLastError();
 }
 
 template <class L, class S, bool FORCE, bool A32>
 hipError_t launch_step_hint(const ColArgs& a, hipStream_t st) {
   switch (a.cache_hint) {
-    case CACHE_NT_STORES: return launch_one<L, S, KMODE_STEP, FORCE, true, false, A32>(a, st);
-    case CACHE_NT_LOADS: return launch_one<L, S, KMODE_STEP, FORCE, false, true, A32>(a, st);
+    case CACHE_NT_STORES:
+      if constexpr (A32) {
+        if (a.resident_planes == 1) return launch_one<L, S, KMODE_STEP, FORCE, true, false, A32, 1>(a, st);
+        if (a.resident_planes == 2) return launch_one<L, S, KMODE_STEP, FORCE, true, false, A32, 2>(a, st);
+      }
+      return launch_one<L, S, KMODE_STEP, FORCE, true, false, A32>(a, st);
+    case CACHE_NT_LOADS:
+      if constexpr (A32) {
+        if (a.resident_planes == 1) return launch_one<L, S, KMODE_STEP, FORCE, false, true, A32, 1>(a, st);
+        if (a.resident_planes == 2) return launch_one<L, S, KMODE_STEP, FORCE, false, true, A32, 2>(a, st);
+      }
+      return launch_one<L, S, KMODE_STEP, FORCE, false, true, A32>(a, st);
   }
   return launch_one<L, S, KMODE_STEP, FORCE, false, false, A32>(a, st);
 }

@@ -129,6 +129,7 @@ struct lbm_solver {
   std::vector<int32_t> bsl_up_src;  // Q per list entry: raw pull sources of the node behind (-1 = bounce back)
   ColArgs cbase_args{};
   int cache_policy = 0;              // 0 plain, 1 nontemporal stores, 2 alternating hints
+  size_t step_cached_bytes = 0;      // metadata and stress data every step reads (kept cached)
 
   // Reverse-pass workspace, kept across lbm_adjoint_backward calls (freed
   // with the solver): state slots in one arena, the two adjoint fields, and
@@ -220,6 +221,8 @@ StepArgs args_for(lbm_solver* s, int in) {
   return a;
 }
 
+constexpr size_t kInfinityCacheBytes = size_t(256) << 20;  // MI355X Infinity Cache (MALL)
+
 ColArgs col_args_for(lbm_solver* s, int in) {
   ColArgs c = s->cbase_args;
   c.fs_in = s->fs[in];
@@ -229,6 +232,20 @@ ColArgs col_args_for(lbm_solver* s, int in) {
   c.cache_hint = s->cache_policy == 0 ? CACHE_PLAIN
                  : s->cache_policy == 1 ? CACHE_NT_STORES
                                         : (in == 0 ? CACHE_NT_LOADS : CACHE_NT_STORES);
+  // Alternating hints with room to spare: the first one or two planes of
+  // both buffers stay cached as well, so the streamed step reads them from
+  // the cache. The budget keeps an 8% margin: the cache is sliced by memory
+  // channel, so a fit much tighter than that depends on how the buffers'
+  // pages fall on the channels (on the 2.85M-node bifurcation two planes,
+  // at 96%, ran 1.7 us faster on most boxes and 9 us slower on one; three,
+  // at 99%, thrashed everywhere).
+  c.resident_planes = 0;
+  if (s->cache_policy == 2) {
+    const size_t plane = static_cast<size_t>(s->npad) * s->esize;
+    const size_t other = s->step_cached_bytes + (s->porosity ? static_cast<size_t>(s->npad) * 4 : 0);
+    for (int k = 2; k >= 1 && !c.resident_planes; --k)
+      if (static_cast<size_t>(s->Q + k) * plane + other <= kInfinityCacheBytes / 25 * 23) c.resident_planes = k;
+  }
   c.porosity = s->porosity;
   c.n_modes = s->n_modes;
   return c;
@@ -627,8 +644,11 @@ int create_columns(lbm_solver* s, const Tables& t) {
   // -> plain accesses; one fits -> alternate the hints so that each step's
   // output stays cached for the next step's input (col_args_for); neither ->
   // stream the stores.
-  const size_t mall = size_t(256) << 20;
+  const size_t mall = kInfinityCacheBytes;
   s->cache_policy = 2 * pop_bytes <= mall ? 0 : (pop_bytes <= mall - mall / 8 ? 2 : 1);
+  s->step_cached_bytes = meta.size() * sizeof(ChunkMeta) + delta.size() * 4 + bbm.size() * 4 +
+                         isrc.size() * 4 + ikind.size() * 12 + nnw1 * 4 * (s->prec == 0 ? 8 : 4) * 2 +
+                         basis.size() * 8;
 #ifdef LBM_CACHE_POLICY
   s->cache_policy = LBM_CACHE_POLICY;  // tuning builds only
 #endif
@@ -1357,6 +1377,7 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
   const bool force = has_force(s);
   ColArgs cf = cb;  // forward recomputation: no stress, no objective
   cf.cache_hint = s->cache_policy == 0 ? CACHE_PLAIN : CACHE_NT_STORES;
+  cf.resident_planes = 0;
   cf.wss_on = 0;
   cf.obj_acc = nullptr;
   cf.obj_w = nullptr;

@@ -311,6 +311,13 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
  * `slots` stored states (<= 0: as many as fit, at most checkpoint_every - 1).
  * The solver state afterwards is the forward pass's final state. */
 int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double* grad_porosity);
+/* Moments tape of the reverse pass (default on): while recomputing a
+ * checkpoint interval the step kernel stores the density and momentum sums
+ * of every node's pulled populations, and the reverse step reads those 4
+ * values instead of re-pulling 19 from the recomputed state (K * 4 values
+ * per node of device memory, taken only where it fits in a quarter of the
+ * free memory). The gradient is the same bit for bit either way. */
+int lbm_adjoint_set_tape(lbm_solver* s, int32_t enable);
 /* The binomial schedule used inside one interval of nsteps steps with
  * `slots` buffers: 5 int32 per action (op 0 advance / 1 reverse, from, to,
  * buffer from, buffer to; buffer -1 = the interval's stored start state,

@@ -36,6 +36,8 @@ struct AdjArgs {
   void* rho_out;         // rho_t
   int32_t rho_cached;    // 1: rho_in was written with cached stores (read it plainly, stream rho_out);
                          // 0: read rho_in streamed, keep rho_out cached
+  const void* tape_in;   // moments (s, jx, jy, jz) of step t's pulled populations, written by the
+                         // recomputation (ColArgs::tape layout), or null: re-pull them from h_{t-1}
   double* own;           // n_nw * Q: stress-equilibrium term (read: step t+1's, written: step t's)
   void* mb;              // (Q + 1) per member (CSR order below): mu_t of an extrapolated
                          // inlet/outlet node and its inlet speed U_b(t)
@@ -56,6 +58,11 @@ struct AdjArgs {
 };
 
 hipError_t launch_adjoint(int Q, int precision, const AdjArgs& a, hipStream_t st);
+// Marks in flag (npad bytes, cleared by the caller) each slot in entries[0, n)
+// and every pull source of it: the state values a reverse step reads outside
+// the moments tape.
+hipError_t launch_spec_flags(int Q, int precision, const ColArgs& c, const int32_t* entries, int32_t n,
+                             uint8_t* flag, hipStream_t st);
 hipError_t launch_adjoint_boundary(int Q, int precision, const AdjArgs& a, hipStream_t st);
 // out[t * n_modes + m] = sum_b gamma[t * n_inlet + b] * basis[m * n_inlet + b]
 hipError_t launch_coef_gradient(const double* gamma, const double* basis, int32_t T, int32_t n_inlet,

@@ -240,9 +240,16 @@ struct AdjKern {
     const bool nw = (m.nw_mask >> l) & 1u;
     const int32_t i = nw ? m.nw_base + rank_below(m.nw_mask, l) : -1;
     const bool in_obj = nw && a.obj_w && a.obj_w[i] != 0.0;
-    // Moments of the step's pre-collision populations G_t (recomputed).
+    // Moments of the step's pre-collision populations G_t: from the tape of
+    // the recomputation, or recomputed here.
     C s, jx, jy, jz;
-    {
+    if (a.mode == ADJ_STEP && a.tape_in) {
+      const C* t = static_cast<const C*>(a.tape_in) + p;
+      s = __builtin_nontemporal_load(t);
+      jx = __builtin_nontemporal_load(t + c.npad);
+      jy = __builtin_nontemporal_load(t + 2 * c.npad);
+      jz = __builtin_nontemporal_load(t + 3 * c.npad);
+    } else {
       C G[Q];
       if (a.mode == ADJ_FINAL) {
 #pragma unroll
@@ -362,14 +369,22 @@ struct AdjKern {
 #pragma clang fp contract(fast)
     const ColArgs& c = a.c;
     const int32_t y = a.tgt[k];
-    C R[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int32_t s = a.tgt_src[int64_t(k) * Q + q];
-      R[q] = K::P::ld(c.fs_in, int64_t(s >= 0 ? q : L::opp[q]) * c.npad + (s >= 0 ? s : y));
-    }
     C s, jx, jy, jz;
-    moments<L, C>(R, s, jx, jy, jz);
+    if (a.tape_in) {  // y is a main slot: its pulled moments are on the tape
+      const C* t = static_cast<const C*>(a.tape_in) + y;
+      s = t[0];
+      jx = t[c.npad];
+      jy = t[2 * c.npad];
+      jz = t[3 * c.npad];
+    } else {
+      C R[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int32_t src = a.tgt_src[int64_t(k) * Q + q];
+        R[q] = K::P::ld(c.fs_in, int64_t(src >= 0 ? q : L::opp[q]) * c.npad + (src >= 0 ? src : y));
+      }
+      moments<L, C>(R, s, jx, jy, jz);
+    }
     const C rR = SHIFT ? C(1) + s : s;
     const C ux = jx / rR, uy = jy / rR, uz = jz / rR;
     C acc[Q];
@@ -479,6 +494,18 @@ __global__ __launch_bounds__(BLOCK) void coef_gradient_kernel(const double* gamm
 }
 
 template <class L, class S>
+__global__ __launch_bounds__(BLOCK) void spec_flag_kernel(ColArgs c, const int32_t* entries, int32_t n,
+                                                          uint8_t* flag) {
+  const int32_t k = blockIdx.x * BLOCK + threadIdx.x;
+  if (k >= n) return;
+  const int32_t p = entries[k];
+  int32_t src[L::Q];
+  colk::ColKern<L, S>::pull_slots(c, p, src);
+  flag[p] = 1;
+  for (int q = 0; q < L::Q; ++q) flag[src[q]] = 1;
+}
+
+template <class L, class S>
 hipError_t launch_a(const AdjArgs& a, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>(a.c.b_blocks + a.c.m_blocks);
   if (!grid) return hipSuccess;
@@ -511,6 +538,18 @@ hipError_t dispatch(int precision, bool boundary, const AdjArgs& a, hipStream_t 
 }
 
 }  // namespace
+
+hipError_t launch_spec_flags(int Q, int precision, const ColArgs& c, const int32_t* entries, int32_t n,
+                             uint8_t* flag, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const dim3 grid((n + BLOCK - 1) / BLOCK), blk(BLOCK);
+  // The decoding reads only the layout, which is the same for every precision.
+  (void)precision;
+  if (Q == 19) hipLaunchKernelGGL((spec_flag_kernel<D3Q19, float>), grid, blk, 0, st, c, entries, n, flag);
+  else if (Q == 27) hipLaunchKernelGGL((spec_flag_kernel<D3Q27, float>), grid, blk, 0, st, c, entries, n, flag);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
 
 hipError_t launch_adjoint(int Q, int precision, const AdjArgs& a, hipStream_t st) {
   if (Q == 19) return dispatch<D3Q19>(precision, false, a, st);

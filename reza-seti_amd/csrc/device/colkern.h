@@ -352,7 +352,53 @@ struct ColKern {
       write_direct(a, p, F);
       return;
     }
+    if constexpr (MODE == KMODE_TAPE) store_tape(a, p, F);
     collide_store<FORCE, NT, A32, KRES>(a, p, F, i);
+    if constexpr (MODE == KMODE_TAPE) store_sparse(a, p, F);
+  }
+
+  // KMODE_TAPE: the post-collision populations F of a flagged slot also go
+  // to the sparse state copy (the few slots whose values the reverse step
+  // still reads from the state: pull sources and own slots of objective
+  // and inlet/outlet nodes).
+  __device__ static void store_sparse(const ColArgs& a, int32_t p, const C* F) {
+    if (!a.spec_out || !a.spec_flag[p]) return;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) P::st_nt(a.spec_out, int64_t(q) * a.npad + p, F[q]);
+  }
+
+  // Pull source slot of every direction of slot p (the node's own slot for a
+  // missing link), decoded from the layout as the pull does.
+  __device__ static void pull_slots(const ColArgs& a, int32_t p, int32_t* src) {
+    const ChunkMeta m = a.meta[p / CHUNK];
+    const int l = p & (CHUNK - 1);
+    if ((m.irr_mask >> l) & 1u) {
+      const int32_t row = m.irr_base + rank_below(m.irr_mask, l);
+      for (int q = 0; q < Q; ++q) {
+        const int32_t s = a.irr_src[int64_t(q) * a.n_irr + row];
+        src[q] = s >= 0 ? s : p;
+      }
+      return;
+    }
+    const uint32_t bm = ((m.bb_mask >> l) & 1u) ? a.bbm[m.bb_base + rank_below(m.bb_mask, l)] : 0u;
+    const Deltas dd = load_deltas(a, segment_of(m, l));
+    const int32_t d[NCOL] = {dd.lo.x, dd.lo.y, dd.lo.z, dd.lo.w, dd.hi.x, dd.hi.y, dd.hi.z, dd.hi.w};
+    for (int q = 0; q < Q; ++q) {
+      const int o = col_of(L::cx[q], L::cy[q]);
+      src[q] = ((bm >> q) & 1u) ? p : (o < 0 ? p : p + d[o]) - L::cz[q];
+    }
+  }
+
+  // Moments of slot p's pulled populations for the adjoint's collision
+  // transpose (read back once, much later: streamed past the caches).
+  __device__ static void store_tape(const ColArgs& a, int32_t p, const C* F) {
+    C s, jx, jy, jz;
+    moments<L, C>(F, s, jx, jy, jz);
+    C* t = static_cast<C*>(a.tape) + p;
+    __builtin_nontemporal_store(s, t);
+    __builtin_nontemporal_store(jx, t + a.npad);
+    __builtin_nontemporal_store(jy, t + 2 * a.npad);
+    __builtin_nontemporal_store(jz, t + 3 * a.npad);
   }
 
   // Unrolled to the fixed mode bound so the basis loads issue together.
@@ -449,6 +495,7 @@ struct ColKern {
       return;
     }
     collide_store<FORCE, NT>(a, p, F, i);
+    if constexpr (MODE == KMODE_TAPE) store_sparse(a, p, F);
   }
 };
 

@@ -122,6 +122,15 @@ struct ColArgs {
   double* obj_acc;        // n_nw: sum over steps of obj_w |tau|^2
   const double* obj_w;    // n_nw: wall-segment weight (0 = outside)
   double* obj_rho;        // n_bslots: sum over steps of the post-rule density
+  // KMODE_TAPE output: density (deviation) sum s and momentum jx, jy, jz of
+  // every main slot's pulled populations, 4 planes of npad in the
+  // arithmetic precision (inlet/outlet slots are not written).
+  void* tape;
+  // KMODE_TAPE: post-collision populations of the slots flagged in spec_flag
+  // (npad bytes) also go to spec_out (a state buffer written only there),
+  // or spec_out is null.
+  void* spec_out;
+  const uint8_t* spec_flag;
 };
 
 hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st);

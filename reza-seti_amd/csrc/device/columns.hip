@@ -161,11 +161,31 @@ hipError_t launch_step(const ColArgs& a, hipStream_t st) {
   return a32 ? launch_step_hint<L, S, FORCE, true>(a, st) : launch_step_hint<L, S, FORCE, false>(a, st);
 }
 
+// Adjoint recomputation with the moments tape, with the step's cache hint
+// (no resident planes).
+template <class L, class S, bool FORCE, bool A32>
+hipError_t launch_tape_hint(const ColArgs& a, hipStream_t st) {
+  switch (a.cache_hint) {
+    case CACHE_NT_STORES: return launch_one<L, S, KMODE_TAPE, FORCE, true, false, A32>(a, st);
+    case CACHE_NT_LOADS: return launch_one<L, S, KMODE_TAPE, FORCE, false, true, A32>(a, st);
+  }
+  return launch_one<L, S, KMODE_TAPE, FORCE, false, false, A32>(a, st);
+}
+
+template <class L, class S, bool FORCE>
+hipError_t launch_tape(const ColArgs& a, hipStream_t st) {
+  if (!a.tape || (a.spec_out && !a.spec_flag)) return hipErrorInvalidValue;
+  const bool a32 = uint64_t(L::Q) * uint64_t(a.npad) * sizeof(S) < (uint64_t(1) << 32);
+  return a32 ? launch_tape_hint<L, S, FORCE, true>(a, st) : launch_tape_hint<L, S, FORCE, false>(a, st);
+}
+
 template <class L, class S>
 hipError_t launch_mode(int kmode, bool force, const ColArgs& a, hipStream_t st) {
   switch (kmode) {
     case KMODE_STEP:
       return force ? launch_step<L, S, true>(a, st) : launch_step<L, S, false>(a, st);
+    case KMODE_TAPE:
+      return force ? launch_tape<L, S, true>(a, st) : launch_tape<L, S, false>(a, st);
     case KMODE_INIT:
       return force ? launch_one<L, S, KMODE_INIT, true, false, false>(a, st)
                    : launch_one<L, S, KMODE_INIT, false, false, false>(a, st);

@@ -16,6 +16,8 @@ constexpr uint32_t SFLAG_PAD = 8u;        // unused slot
 constexpr int KMODE_STEP = 0;         // stream + BC + WSS + collide
 constexpr int KMODE_INIT = 1;         // collide f given directly (set / create)
 constexpr int KMODE_MATERIALIZE = 2;  // stream + BC only, write post-BC f
+constexpr int KMODE_TAPE = 3;         // KMODE_STEP without stress, plus the moments of the pulled
+                                       // populations (adjoint recomputation; column family only)
 
 constexpr int BLOCK = 256;
 constexpr int MAX_MODES = 16;

@@ -145,6 +145,12 @@ struct lbm_solver {
     double* galpha = nullptr;
     int32_t *tgt = nullptr, *tgt_src = nullptr, *tgt_off = nullptr, *tgt_inl = nullptr, *member_of = nullptr;
     int32_t n_tgt = -1;  // -1: tables not built yet
+    char* tape = nullptr;  // moments tape of one checkpoint interval
+    size_t tape_bytes = 0;
+    uint8_t* spec_flag = nullptr;    // npad: slots kept in the sparse state copies
+    int32_t* spec_entries = nullptr;  // slots whose pulls the reverse step re-reads
+    size_t spec_cap = 0;
+    bool tape_on = true;   // lbm_adjoint_set_tape
   } adjw;
 
   // Adjoint forward record (lbm_adjoint_forward).
@@ -159,6 +165,7 @@ struct lbm_solver {
     double* obj_w = nullptr;          // n_nw
     double* obj_acc = nullptr;        // n_nw
     double* obj_rho = nullptr;        // n_bslots
+    std::vector<int32_t> obj_slots;   // slots of the near-wall nodes with a nonzero stress weight
   } adj;
 };
 
@@ -194,7 +201,7 @@ void free_all(lbm_solver* s) {
   {
     auto& W = s->adjw;
     void* ptrs[] = {W.arena, W.ckpt_arena, W.rho[0], W.rho[1], W.mb, W.own, W.galpha, W.tgt, W.tgt_src, W.tgt_off,
-                    W.tgt_inl, W.member_of};
+                    W.tgt_inl, W.member_of, W.tape, W.spec_flag, W.spec_entries};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     W = lbm_solver::AdjWork{};
@@ -1181,6 +1188,9 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
   std::vector<double> w(nnw, 0.0);
   for (size_t i = 0; i < s->nw_rows.size(); ++i)
     w[s->nw_rows[i]] = (obj->wss_mask && !obj->wss_mask[i]) ? 0.0 : obj->wss_weight;
+  A.obj_slots.clear();
+  for (size_t i = 0; i < s->nw_rows.size(); ++i)
+    if (w[s->nw_rows[i]] != 0.0) A.obj_slots.push_back(s->nw_slots[i]);
   HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_w), nnw * 8));
   HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_acc), nnw * 8));
   HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_rho), nb * 8));
@@ -1350,6 +1360,32 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
     W.n_tgt = static_cast<int32_t>(h_tgt.size());
   }
 
+  // Moments tape: while recomputing a checkpoint interval, the step kernel
+  // also stores the moments of every slot's pulled populations (4 values
+  // instead of 19), so the reverse step of each recomputed step reads them
+  // instead of re-pulling its input state. Steps that the recomputation
+  // does not execute (the last one of each interval) re-pull. Taken only
+  // where it fits in a quarter of the free memory.
+  const size_t tape_step = static_cast<size_t>(4) * npad * aes;
+  char* tape = nullptr;
+  if (W.tape_on && s->family == 1) {
+    const size_t tneed = static_cast<size_t>(K) * tape_step;
+    if (W.tape_bytes < tneed) {
+      if (W.tape) (void)hipFree(W.tape);
+      W.tape = nullptr;
+      W.tape_bytes = 0;
+      size_t free_b = 0, total_b = 0;
+      HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+      if (tneed <= free_b / 4) {
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&W.tape), tneed));
+        W.tape_bytes = tneed;
+      }
+    }
+    tape = W.tape;
+  }
+  std::vector<char> tape_ok(static_cast<size_t>(std::max(K, 1)), 0);
+  int32_t l_cur = 0;  // first step of the interval being reversed
+
   // State buffers for the binomial schedule: `slots` of them plus a
   // scratch state, in one arena that later calls reuse.
   const int32_t max_slots = std::max<int32_t>(K - 1, 0);
@@ -1389,7 +1425,13 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
       c.fs_in = src;
       c.fs_out = out;
       for (int m = 0; m < MAX_MODES; ++m) c.coef[m] = m < M ? A.coef[int64_t(k) * M + m] : 0.0;
-      HIP_TRY(launch_columns(Q, s->prec, KMODE_STEP, force, c, s->stream));
+      if (tape) {  // step k + 1 of the interval: tape entry k - l_cur
+        c.tape = tape + static_cast<size_t>(k - l_cur) * tape_step;
+        HIP_TRY(launch_columns(Q, s->prec, KMODE_TAPE, force, c, s->stream));
+        tape_ok[k - l_cur] = 1;
+      } else {
+        HIP_TRY(launch_columns(Q, s->prec, KMODE_STEP, force, c, s->stream));
+      }
       src = out;
     }
     return LBM_OK;
@@ -1419,6 +1461,7 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
     for (int m = 0; m < MAX_MODES; ++m) x.c.coef[m] = m < M ? A.coef[int64_t(t - 1) * M + m] : 0.0;
     x.rho_in = t == T ? nullptr : rho[(t + 1) & 1];
     x.rho_out = rho[t & 1];
+    x.tape_in = (tape && tape_ok[t - 1 - l_cur]) ? tape + static_cast<size_t>(t - 1 - l_cur) * tape_step : nullptr;
     // Alternate the cache hints so that every other launch reads an adjoint
     // field still held in the Infinity Cache: odd steps write rho[1] with
     // cached stores (and stream their input), even steps read it plainly
@@ -1429,8 +1472,65 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
     HIP_TRY(launch_adjoint_boundary(Q, s->prec, x, s->stream));
     return LBM_OK;
   };
+  // Sparse path: with the tape and a buffer for every step of an interval,
+  // the recomputation runs through each interval once on the two state
+  // buffers, with the forward step's alternating cache hints, and keeps of
+  // each state only the slots the reverse step still reads from it (pull
+  // sources and own slots of objective and inlet/outlet nodes).
+  const bool sparse = tape && max_slots > 0 && slots >= max_slots;
+  if (sparse) {
+    std::vector<int32_t> ent(A.obj_slots);
+    for (size_t i = 0; i < s->bsl_slot.size(); ++i) {
+      const int32_t u = s->bsl_up[i], k = s->bsl_kind[i];
+      const bool nee = u >= 0 && ((k == KIND_INLET && s->prm.inlet_mode == LBM_INLET_NEE_VELOCITY) ||
+                                  (k == KIND_OUTLET && s->prm.outlet_mode == LBM_OUTLET_NEE_PRESSURE));
+      ent.push_back(s->bsl_slot[i]);
+      if (nee) ent.push_back(s->slot_of[u]);
+    }
+    if (W.spec_cap < ent.size()) {
+      if (W.spec_entries) (void)hipFree(W.spec_entries);
+      W.spec_entries = nullptr;
+      W.spec_cap = 0;
+      HIP_TRY(hipMalloc(reinterpret_cast<void**>(&W.spec_entries), ent.size() * 4));
+      W.spec_cap = ent.size();
+    }
+    HIP_TRY(keep(&W.spec_flag, static_cast<size_t>(npad)));
+    if (!ent.empty())
+      HIP_TRY(hipMemcpyAsync(W.spec_entries, ent.data(), ent.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(W.spec_flag, 0, static_cast<size_t>(npad), s->stream));
+    HIP_TRY(launch_spec_flags(Q, s->prec, cb, W.spec_entries, static_cast<int32_t>(ent.size()), W.spec_flag,
+                              s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));  // ent is freed on return
+  }
   for (int32_t j = A.nck - 2; j >= 0; --j) {
     const int32_t l = j * K, n = std::min(K, T - l);
+    l_cur = l;
+    std::fill(tape_ok.begin(), tape_ok.end(), 0);
+    if (sparse) {
+      void* src = A.ckpt[j];
+      for (int32_t k = l; k + 1 < l + n; ++k) {
+        void* out = s->fs[(k - l) & 1];
+        ColArgs c = cf;
+        c.fs_in = src;
+        c.fs_out = out;
+        for (int m = 0; m < MAX_MODES; ++m) c.coef[m] = m < M ? A.coef[int64_t(k) * M + m] : 0.0;
+        if (s->cache_policy == 2) c.cache_hint = out == s->fs[1] ? CACHE_NT_LOADS : CACHE_NT_STORES;
+        c.tape = tape + static_cast<size_t>(k - l) * tape_step;
+        c.spec_out = slot_buf[k - l];
+        c.spec_flag = W.spec_flag;
+        HIP_TRY(launch_columns(Q, s->prec, KMODE_TAPE, force, c, s->stream));
+        tape_ok[k - l] = 1;
+        src = out;
+      }
+      // h_{t-1}: the checkpoint, the last recomputed state in full (its step
+      // has no tape: the recomputation does not execute it), or a sparse copy.
+      for (int32_t t = l + n; t > l; --t) {
+        void* h = (t - 1 == l) ? A.ckpt[j] : (t == l + n ? src : slot_buf[t - 2 - l]);
+        const int rc = reverse(t, h);
+        if (rc) return rc;
+      }
+      continue;
+    }
     auto buffer = [&](int32_t id) -> void* {
       return id == REV_BASE ? A.ckpt[j] : (id == REV_SCRATCH ? scratch : slot_buf[id]);
     };
@@ -1465,6 +1565,12 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (grad_porosity)
     for (int64_t n = 0; n < s->N; ++n) grad_porosity[n] = ga[s->slot_of[n]];
+  return LBM_OK;
+}
+
+int lbm_adjoint_set_tape(lbm_solver* s, int32_t enable) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  s->adjw.tape_on = enable != 0;
   return LBM_OK;
 }
 

@@ -125,6 +125,7 @@ SIGNATURES = [
     ("lbm_copy_probe", [C.c_int32, C.c_int64, C.c_int32, _P], C.c_int),
     ("lbm_adjoint_forward", [_P, C.c_int32, _P, _P, C.c_int32, _P], C.c_int),
     ("lbm_adjoint_backward", [_P, C.c_int32, _P, _P], C.c_int),
+    ("lbm_adjoint_set_tape", [_P, C.c_int32], C.c_int),
     ("lbm_checkpoint_schedule", [C.c_int32, C.c_int32, _P, C.c_int64, _P], C.c_int),
     ("lbm_last_error", [], C.c_char_p),
 ]
@@ -422,8 +423,11 @@ class Solver:
         self._adj_modes = self.n_modes
         return J.value
 
-    def adjoint_backward(self, slots=0):
-        """Reverse pass: (dJ/dcoef [steps, modes], dJ/dalpha [nodes])."""
+    def adjoint_backward(self, slots=0, tape=True):
+        """Reverse pass: (dJ/dcoef [steps, modes], dJ/dalpha [nodes]).
+        tape=False re-pulls every reverse step's moments instead of reading
+        the recomputation's moments tape (same result bit for bit)."""
+        _check(lib().lbm_adjoint_set_tape(self._h, 1 if tape else 0))
         gc = np.zeros((self._adj_steps, self._adj_modes))
         ga = np.zeros(self.n)
         _check(lib().lbm_adjoint_backward(self._h, int(slots), _ptr(gc), _ptr(ga)))

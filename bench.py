@@ -35,16 +35,16 @@ WO, PERIOD, U_MEAN, U_AMP = 4.0, 4000, 0.05, 0.025
 WSS_EVERY = 1  # wall shear stress and its time sums every step, as Simulation::Iterate does
 
 
-def womersley_basis(r):
+def womersley_basis(r, wo=None):
     """Three inlet modes: mean Poiseuille (mean 1) and the real / imaginary
-    parts of the Womersley profile normalised to unit mean over the inlet
-    nodes, so u = U_mean b0 + U_amp (b1 cos wt + b2 sin wt)."""
+    parts of the Womersley profile (Womersley number wo) normalised to unit
+    mean over the inlet nodes, so u = U_mean b0 + U_amp (b1 cos wt + b2 sin wt)."""
     from scipy.special import jv
 
     r = np.clip(r, 0.0, 1.0)
     b0 = 2.0 * (1.0 - r * r)
     b0 /= b0.mean()
-    a = (1j ** 1.5) * WO
+    a = (1j ** 1.5) * (WO if wo is None else wo)
     P = 1.0 - jv(0, a * r) / jv(0, a)
     P /= P.mean()
     return np.stack([b0, P.real, -P.imag])
@@ -239,6 +239,67 @@ def run_c2(steps, warmup=20):
     return out
 
 
+C4_SCALE, C4_WO, C4_PERIOD, C4_WSS_EVERY = 4.3, 5.0, 8000, 100
+
+
+def run_c4(steps, warmup=20):
+    """Large single-GPU share: the tortuous vessel (Catmull-Rom centreline
+    through a seeded 12-point random walk, radius 20-40 and a saccular bulge
+    of radius 60, all lengths x4.3: ~300M nodes), D3Q19 FP32-shifted,
+    Womersley inlet (Wo 5, period 8000 steps, U_mean 0.05), NEE zero-pressure
+    outlet, bounce-back walls, wall shear stress every 100 steps. Returns the
+    step rate and its roofline fraction, with the set-up times."""
+    t0 = time.perf_counter()
+    fluid, origin, info = workloads.c4_vessel(scale=C4_SCALE)
+    t1 = time.perf_counter()
+    print(f"# large share: labels {fluid.shape} in {t1 - t0:.1f}s", file=sys.stderr, flush=True)
+    nz = fluid.shape[2]
+    inl = lbm.plane_block("z", float(origin[2]), (0.0, 0.0, -1.0))
+    out_b = lbm.plane_block("z", float(origin[2] + nz - 1), (0.0, 0.0, 1.0))
+    geom = lbm.Geometry.from_labels(fluid, origin, 1.0, inlets=[inl], outlets=[out_b], triangles=info["triangles"],
+                                    normals=2, nthreads=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    grid = tuple(int(v) for v in fluid.shape)
+    del fluid
+    t2 = time.perf_counter()
+    print(f"# large share: {geom.n} nodes, geometry {t2 - t1:.1f}s", file=sys.stderr, flush=True)
+    ijk, kind, _ = geom.nodes()
+    ii = np.where(kind == lbm.KIND_INLET)[0]
+    c0, r0 = info["centreline"][0], float(info["radius"][0])
+    r = np.hypot(origin[0] + ijk[ii, 0] - c0[0], origin[1] + ijk[ii, 1] - c0[1]) / r0
+    del ijk, kind
+    basis = womersley_basis(r, C4_WO)
+    w = 2.0 * math.pi / C4_PERIOD
+    tau = 3.0 * r0 * r0 * w / (C4_WO * C4_WO) + 0.5
+
+    def coef(t0_, n):
+        t = np.arange(t0_, t0_ + n, dtype=np.float64)
+        return np.stack([np.full(n, U_MEAN), U_AMP * np.cos(w * t), U_AMP * np.sin(w * t)], 1)
+
+    s = lbm.Solver(geom, precision=lbm.PREC_F32, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NEE_VELOCITY,
+                   outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=tau, dt=1e-5, parity_checks=0, wss_every=C4_WSS_EVERY)
+    s.set_inlet_profile(basis)
+    t3 = time.perf_counter()
+    print(f"# large share: solver {t3 - t2:.1f}s", file=sys.stderr, flush=True)
+    s.step_coefficients(coef(0, warmup))
+    s.synchronize()
+    ta = time.perf_counter()
+    s.step_coefficients(coef(warmup, steps))
+    s.synchronize()
+    wall = time.perf_counter() - ta
+    ev_ms, _ = s.last_batch_ms()
+    n = geom.n
+    achieved = n * 152.0 / (ev_ms / 1e3 / steps) / 1e9
+    res = {"workload": f"tortuous vessel x{C4_SCALE} (12-point walk, radius {20 * C4_SCALE:.0f}-{40 * C4_SCALE:.0f}, "
+                       f"bulge {60 * C4_SCALE:.0f}), D3Q19 FP32-shifted, Womersley Wo {C4_WO} period {C4_PERIOD}, "
+                       f"WSS every {C4_WSS_EVERY} steps",
+           "nodes": n, "grid": grid, "triangles": info["n_triangles"], "steps": steps, "tau": round(tau, 4),
+           "f32_mlups": round(n * steps / wall / 1e6, 1), "roofline_frac": round(achieved / PEAK_HBM_GBS, 4),
+           "achieved_gbs": round(achieved, 1), "launch_us": round(ev_ms * 1e3 / steps, 2),
+           "setup_s": {"labels": round(t1 - t0, 1), "geometry": round(t2 - t1, 1), "solver": round(t3 - t2, 1)}}
+    s.close()
+    return res
+
+
 def cpu_baseline(geom, budget_s=12.0):
     """The oracle (FP64 restatement of the reference step, OpenMP) on the host
     cores, on a bounded sample of the same workload."""
@@ -293,6 +354,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--c2-steps", type=int, default=500)
     ap.add_argument("--adjoint-steps", type=int, default=ADJ_STEPS)
+    ap.add_argument("--c4", action="store_true", help="also run the ~300M-node large share (minutes of set-up)")
+    ap.add_argument("--c4-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole lattice on a single MI355X "
@@ -359,6 +422,10 @@ def main():
         out["d3q27_stenosis"] = run_c2(args.c2_steps)
         print(f"# D3Q27 stenosis: f32 {out['d3q27_stenosis']['f32_mlups']:.0f} MLUPS, "
               f"f64 {out['d3q27_stenosis']['f64_mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    if args.c4:
+        out["large_share"] = run_c4(args.c4_steps)
+        print(f"# large share: {out['large_share']['nodes']} nodes, {out['large_share']['f32_mlups']:.0f} MLUPS, "
+              f"roofline {out['large_share']['roofline_frac']:.3f}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(geom)
     print(json.dumps(out))

@@ -440,3 +440,136 @@ def branch_segment_mask(points, origin, direction, s0, length, radius):
     s = rel @ d
     dist = np.linalg.norm(rel - s[:, None] * d[None, :], axis=1)
     return (s >= s0) & (s < s0 + length) & (dist <= radius)
+
+
+# ------------------------------------------------------ large single-GPU share
+def _catmull_rom(P, samples_per_seg):
+    """Uniform Catmull-Rom curve through the rows of P (phantom end points
+    reflected), sampled samples_per_seg times per segment; returns points."""
+    P = np.asarray(P, dtype=np.float64)
+    Q = np.vstack([2 * P[0] - P[1], P, 2 * P[-1] - P[-2]])
+    out = []
+    t = np.linspace(0.0, 1.0, samples_per_seg, endpoint=False)[:, None]
+    for i in range(1, len(Q) - 2):
+        p0, p1, p2, p3 = Q[i - 1], Q[i], Q[i + 1], Q[i + 2]
+        out.append(0.5 * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t * t +
+                          (-p0 + 3 * p1 - 3 * p2 + p3) * t * t * t))
+    out.append(P[-1][None, :])
+    return np.vstack(out)
+
+
+def c4_vessel(scale=4.5, seed=5, device=None, sample_step=2.0, slab=32):
+    """Tortuous vessel of the large single-GPU share: a Catmull-Rom
+    centreline through a 12-point random walk (step 80 scale lu, a persistent
+    walk heading +z whose first and last steps run along z), radius varying
+    between 20 and 40 scale lu along the arc, and a spherical saccular bulge
+    of radius 60 scale lu on the side of the vessel at 55% of its length.
+    scale = 4.5 gives about 300M nodes at pitch 1 (the 80/20-40/60 lu sizes
+    as such give about 3.3M).
+
+    Labels are the union of balls of radius r(s) centred on the centreline
+    sampled every sample_step lu (a swept tube), and the bulge ball, on the
+    lattice x = origin + (i, j, k); the box is cut at the end points' z, so
+    the vessel's ends lie on the z faces (inlet at z min, normal -z; outlet
+    at z max, normal +z). Computed slab by slab with torch on `device`
+    (default: the GPU when there is one). Returns (fluid uint8 [nx, ny, nz],
+    origin, info); info holds the triangles of a swept-ring surface mesh and
+    the bulge sphere (for the near-wall normals), the plane blocks and sizes."""
+    import torch
+
+    if device is None:
+        device = "cuda" if torch.cuda.is_available() else "cpu"
+    rng = np.random.default_rng(seed)
+    step, rmin, rmax, rb = 80.0 * scale, 20.0 * scale, 40.0 * scale, 60.0 * scale
+    P = [np.zeros(3), np.array([0.0, 0.0, step])]
+    d = np.array([0.0, 0.0, 1.0])
+    for _ in range(9):
+        d = _unit(d + 0.45 * rng.normal(size=3))
+        d[2] = max(d[2], 0.55)
+        d = _unit(d)
+        P.append(P[-1] + step * d)
+    P.append(P[-1] + np.array([0.0, 0.0, step]))
+    P = np.array(P)
+    dense = _catmull_rom(P, 400)
+    seg = np.linalg.norm(np.diff(dense, axis=0), axis=1)
+    s_dense = np.concatenate([[0.0], np.cumsum(seg)])
+    length = float(s_dense[-1])
+    s = np.arange(0.0, length, sample_step)
+    s = np.append(s, length)
+    C = np.stack([np.interp(s, s_dense, dense[:, a]) for a in range(3)], axis=1)
+    phase = rng.uniform(0, 2 * np.pi)
+    R = rmin + (rmax - rmin) * (0.5 + 0.5 * np.sin(2 * np.pi * 1.5 * s / length + phase))
+    # Bulge: centre off the axis, perpendicular to the tangent, so that the
+    # sphere overlaps the wall (a neck) and bulges out of the side.
+    kb = int(np.searchsorted(s, 0.55 * length))
+    tb = _unit(C[min(kb + 1, len(C) - 1)] - C[max(kb - 1, 0)])
+    nb = _unit(np.cross(tb, rng.normal(size=3)))
+    cb = C[kb] + nb * (R[kb] + 0.35 * rb)
+    z0, z1 = float(C[0, 2]), float(C[-1, 2])
+    lo = np.minimum((C - R[:, None]).min(0), cb - rb) - 2.0
+    hi = np.maximum((C + R[:, None]).max(0), cb + rb) + 2.0
+    origin = np.array([np.floor(lo[0]) + 0.37, np.floor(lo[1]) + 0.21, z0])
+    nx = int(np.ceil(hi[0] - origin[0])) + 1
+    ny = int(np.ceil(hi[1] - origin[1])) + 1
+    nz = int(np.floor(z1 - z0)) + 1
+    fluid = np.zeros((nx, ny, nz), np.uint8)
+    dev = torch.device(device)
+    Ct = torch.tensor(C, dtype=torch.float32, device=dev)
+    Rt = torch.tensor(R, dtype=torch.float32, device=dev)
+    cbt = torch.tensor(cb, dtype=torch.float32, device=dev)
+    for k0 in range(0, nz, slab):
+        k1 = min(nz, k0 + slab)
+        za, zb = origin[2] + k0, origin[2] + k1 - 1
+        sel = np.where((C[:, 2] + R >= za) & (C[:, 2] - R <= zb))[0]
+        boxes = []
+        if sel.size:
+            boxes.append((C[sel, :2] - R[sel, None]).min(0), )
+            blo = (C[sel, :2] - R[sel, None]).min(0)
+            bhi = (C[sel, :2] + R[sel, None]).max(0)
+        else:
+            blo, bhi = None, None
+        if cb[2] + rb >= za and cb[2] - rb <= zb:
+            blo = cb[:2] - rb if blo is None else np.minimum(blo, cb[:2] - rb)
+            bhi = cb[:2] + rb if bhi is None else np.maximum(bhi, cb[:2] + rb)
+        if blo is None:
+            continue
+        i0 = max(0, int(np.floor(blo[0] - origin[0]))); i1 = min(nx, int(np.ceil(bhi[0] - origin[0])) + 1)
+        j0 = max(0, int(np.floor(blo[1] - origin[1]))); j1 = min(ny, int(np.ceil(bhi[1] - origin[1])) + 1)
+        xs = torch.arange(i0, i1, device=dev, dtype=torch.float32) + float(origin[0])
+        ys = torch.arange(j0, j1, device=dev, dtype=torch.float32) + float(origin[1])
+        zs = torch.arange(k0, k1, device=dev, dtype=torch.float32) + float(origin[2])
+        X, Y, Z = torch.meshgrid(xs, ys, zs, indexing="ij")
+        pts = torch.stack([X.reshape(-1), Y.reshape(-1), Z.reshape(-1)], dim=1)
+        inside = ((pts - cbt) ** 2).sum(1) < rb * rb
+        if sel.size:
+            cs, rs = Ct[torch.as_tensor(sel, device=dev)], Rt[torch.as_tensor(sel, device=dev)]
+            for a in range(0, pts.shape[0], 1 << 20):
+                d2 = torch.cdist(pts[a:a + (1 << 20)], cs) - rs[None, :]
+                inside[a:a + (1 << 20)] |= d2.amin(dim=1) < 0
+        fluid[i0:i1, j0:j1, k0:k1] = inside.reshape(i1 - i0, j1 - j0, k1 - k0).to(torch.uint8).cpu().numpy()
+    # Surface mesh for the near-wall normals: rings around the centreline
+    # (parallel-transported frames, outward triangle order) and the bulge sphere.
+    ring_every, nseg = max(1, int(round(8.0 / sample_step))), 64
+    idx = np.arange(0, len(C), ring_every)
+    T = np.gradient(C, axis=0)
+    T = T / np.linalg.norm(T, axis=1)[:, None]
+    u = _unit(np.cross(T[0], [1.0, 0.0, 0.0]) if abs(T[0, 0]) < 0.9 else np.cross(T[0], [0.0, 1.0, 0.0]))
+    rings = []
+    for k in idx:
+        t = T[k]
+        u = _unit(u - np.dot(u, t) * t)
+        v = np.cross(t, u)
+        th = 2 * np.pi * (np.arange(nseg) + 0.5) / nseg
+        rings.append(C[k] + R[k] * (np.cos(th)[:, None] * u + np.sin(th)[:, None] * v))
+    rings = np.array(rings)
+    a = rings[:-1, :, :]
+    b = np.roll(rings[:-1], -1, axis=1)
+    c = rings[1:, :, :]
+    dd = np.roll(rings[1:], -1, axis=1)
+    tube = np.concatenate([np.stack([a, b, c], axis=2).reshape(-1, 3, 3), np.stack([b, dd, c], axis=2).reshape(-1, 3, 3)])
+    sph, _ = icosphere(cb, rb, subdiv=3, rotation_seed=seed)
+    tris = np.concatenate([tube, sph])
+    info = dict(scale=scale, length=length, centreline=C, radius=R, bulge_center=cb, bulge_radius=rb,
+                control_points=P, grid=(nx, ny, nz), z_range=(z0, z1), triangles=tris.reshape(-1, 9),
+                n_triangles=len(tris), inlet_normal=(0.0, 0.0, -1.0), outlet_normal=(0.0, 0.0, 1.0))
+    return fluid, origin, info

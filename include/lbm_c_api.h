@@ -264,6 +264,14 @@ int lbm_step_timed(lbm_solver* s, int32_t nsteps, double* ms);
  * lbm_step / lbm_step_coefficients / lbm_step_timed batch, and its step
  * count. Waits for that batch to finish. */
 int lbm_last_batch_time(lbm_solver* s, double* ms, int32_t* nsteps);
+/* Split timing (no reference counterpart; measurement only): with it on,
+ * every later batch also records an event pair around each of its first 16
+ * launches that fold the stress ring into the FP64 accumulators, and
+ * lbm_last_batch_kernel_time returns the time of all those launches (their
+ * timed mean times their count, other_ms) and the rest of the batch's time
+ * (the step kernel's share, step_ms). */
+int lbm_set_kernel_timing(lbm_solver* s, int32_t enable);
+int lbm_last_batch_kernel_time(lbm_solver* s, double* step_ms, double* other_ms);
 
 int lbm_synchronize(lbm_solver* s);
 void lbm_destroy(lbm_solver* s);

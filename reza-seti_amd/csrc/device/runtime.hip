@@ -104,6 +104,11 @@ struct lbm_solver {
   std::vector<double> last_set;      // f given to the last set (node-major), empty = w
   int n_modes = 1;
   int32_t batch_steps = 0;           // steps of the last lbm_step* call (timed by ev0/ev1)
+  bool split_timing = false;         // lbm_set_kernel_timing: event pairs around stress flushes
+  bool in_batch = false;
+  std::vector<hipEvent_t> flush_ev;  // pairs, the first flush_used of them recorded in the last batch
+  size_t flush_used = 0;
+  int64_t flush_count = 0;           // stress flushes launched in the last batch
   double last_coef[MAX_MODES] = {0};
   StepArgs base{};
 
@@ -214,6 +219,8 @@ void free_all(lbm_solver* s) {
     if (p) (void)hipFree(p);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  for (hipEvent_t e : s->flush_ev) (void)hipEventDestroy(e);
+  s->flush_ev.clear();
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -303,7 +310,24 @@ bool has_force(const lbm_solver* s) {
 // Folds the pending stress ring slots into the FP64 accumulators.
 int flush_wss(lbm_solver* s) {
   if (s->family != 1 || s->ring_count == 0) return LBM_OK;
+  // The first kTimedFlushes flushes of a batch are timed (their mean stands
+  // for all of them; events on every flush would slow the batch itself).
+  constexpr size_t kTimedFlushes = 16;
+  const bool timed = s->split_timing && s->in_batch && s->flush_used < 2 * kTimedFlushes;
+  if (s->in_batch) s->flush_count++;
+  if (timed) {
+    while (s->flush_ev.size() < s->flush_used + 2) {
+      hipEvent_t e = nullptr;
+      HIP_TRY(hipEventCreate(&e));
+      s->flush_ev.push_back(e);
+    }
+    HIP_TRY(hipEventRecord(s->flush_ev[s->flush_used], s->stream));
+  }
   HIP_TRY(launch_wss_flush(s->prec, s->cbase_args, s->ring_count, s->stream));
+  if (timed) {
+    HIP_TRY(hipEventRecord(s->flush_ev[s->flush_used + 1], s->stream));
+    s->flush_used += 2;
+  }
   s->ring_count = 0;
   return LBM_OK;
 }
@@ -320,6 +344,13 @@ int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
   if (nsteps < 0) return fail(LBM_ERR_ARG, "nsteps < 0");
   HIP_TRY(hipSetDevice(s->prm.device));
   const bool force = has_force(s);
+  s->flush_used = 0;
+  s->flush_count = 0;
+  s->in_batch = true;
+  struct BatchEnd {
+    lbm_solver* s;
+    ~BatchEnd() { s->in_batch = false; }
+  } batch_end{s};
   HIP_TRY(hipEventRecord(s->ev0, s->stream));
   for (int32_t t = 0; t < nsteps; ++t) {
     StepArgs a = args_for(s, s->cur);
@@ -974,6 +1005,30 @@ int lbm_last_batch_time(lbm_solver* s, double* ms, int32_t* nsteps) {
   HIP_TRY(hipEventElapsedTime(&e, s->ev0, s->ev1));
   *ms = e;
   if (nsteps) *nsteps = s->batch_steps;
+  return LBM_OK;
+}
+
+int lbm_set_kernel_timing(lbm_solver* s, int32_t enable) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  s->split_timing = enable != 0;
+  return LBM_OK;
+}
+
+int lbm_last_batch_kernel_time(lbm_solver* s, double* step_ms, double* other_ms) {
+  if (!s || !step_ms || !other_ms) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  HIP_TRY(hipEventSynchronize(s->ev1));
+  float e = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&e, s->ev0, s->ev1));
+  double other = 0.0;
+  for (size_t i = 0; i + 1 < s->flush_used; i += 2) {
+    float f = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&f, s->flush_ev[i], s->flush_ev[i + 1]));
+    other += f;
+  }
+  if (s->flush_used) other *= double(s->flush_count) / double(s->flush_used / 2);
+  *other_ms = other;
+  *step_ms = double(e) - other;
   return LBM_OK;
 }
 

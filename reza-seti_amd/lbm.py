@@ -119,6 +119,8 @@ SIGNATURES = [
     ("lbm_reduce", [_P, _P], C.c_int),
     ("lbm_step_timed", [_P, C.c_int32, _P], C.c_int),
     ("lbm_last_batch_time", [_P, _P, _P], C.c_int),
+    ("lbm_set_kernel_timing", [_P, C.c_int32], C.c_int),
+    ("lbm_last_batch_kernel_time", [_P, _P, _P], C.c_int),
     ("lbm_synchronize", [_P], C.c_int),
     ("lbm_destroy", [_P], None),
     ("lbm_get_solver_info", [_P, _P], C.c_int),
@@ -351,6 +353,16 @@ class Solver:
         n = C.c_int32()
         _check(lib().lbm_last_batch_time(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def set_kernel_timing(self, on=True):
+        """Time the stress-flush launches of later batches separately."""
+        _check(lib().lbm_set_kernel_timing(self._h, 1 if on else 0))
+
+    def last_batch_kernel_ms(self):
+        """(step kernel ms, stress-flush ms) of the last batch (needs set_kernel_timing)."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib().lbm_last_batch_kernel_time(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_inlet_profile(self, basis):
         basis = np.ascontiguousarray(np.atleast_2d(basis), dtype=np.float64)

@@ -271,6 +271,10 @@ int lbm_last_batch_time(lbm_solver* s, double* ms, int32_t* nsteps);
  * timed mean times their count, other_ms) and the rest of the batch's time
  * (the step kernel's share, step_ms). */
 int lbm_set_kernel_timing(lbm_solver* s, int32_t enable);
+/* Column family: take the 64-bit element offsets that states of 4 GiB or
+ * more need on a state of any size (no reference counterpart; it lets a
+ * small case check that path against the 32-bit one, bit for bit). */
+int lbm_set_wide_offsets(lbm_solver* s, int32_t enable);
 int lbm_last_batch_kernel_time(lbm_solver* s, double* step_ms, double* other_ms);
 
 int lbm_synchronize(lbm_solver* s);

@@ -131,6 +131,7 @@ struct ColArgs {
   // or spec_out is null.
   void* spec_out;
   const uint8_t* spec_flag;
+  int32_t wide;  // 1: 64-bit element offsets whatever the state size (the path of states of 4 GiB or more)
 };
 
 hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st);

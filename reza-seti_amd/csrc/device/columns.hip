@@ -157,7 +157,7 @@ hipError_t launch_step_hint(const ColArgs& a, hipStream_t st) {
 // 32-bit byte offsets when the whole state buffer spans fewer than 2^32 bytes.
 template <class L, class S, bool FORCE>
 hipError_t launch_step(const ColArgs& a, hipStream_t st) {
-  const bool a32 = uint64_t(L::Q) * uint64_t(a.npad) * sizeof(S) < (uint64_t(1) << 32);
+  const bool a32 = !a.wide && uint64_t(L::Q) * uint64_t(a.npad) * sizeof(S) < (uint64_t(1) << 32);
   return a32 ? launch_step_hint<L, S, FORCE, true>(a, st) : launch_step_hint<L, S, FORCE, false>(a, st);
 }
 
@@ -175,7 +175,7 @@ hipError_t launch_tape_hint(const ColArgs& a, hipStream_t st) {
 template <class L, class S, bool FORCE>
 hipError_t launch_tape(const ColArgs& a, hipStream_t st) {
   if (!a.tape || (a.spec_out && !a.spec_flag)) return hipErrorInvalidValue;
-  const bool a32 = uint64_t(L::Q) * uint64_t(a.npad) * sizeof(S) < (uint64_t(1) << 32);
+  const bool a32 = !a.wide && uint64_t(L::Q) * uint64_t(a.npad) * sizeof(S) < (uint64_t(1) << 32);
   return a32 ? launch_tape_hint<L, S, FORCE, true>(a, st) : launch_tape_hint<L, S, FORCE, false>(a, st);
 }
 

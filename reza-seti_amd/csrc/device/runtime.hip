@@ -1008,6 +1008,12 @@ int lbm_last_batch_time(lbm_solver* s, double* ms, int32_t* nsteps) {
   return LBM_OK;
 }
 
+int lbm_set_wide_offsets(lbm_solver* s, int32_t enable) {
+  if (!s) return fail(LBM_ERR_ARG, "null argument");
+  s->cbase_args.wide = enable != 0 ? 1 : 0;
+  return LBM_OK;
+}
+
 int lbm_set_kernel_timing(lbm_solver* s, int32_t enable) {
   if (!s) return fail(LBM_ERR_ARG, "null argument");
   s->split_timing = enable != 0;

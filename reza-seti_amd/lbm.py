@@ -120,6 +120,7 @@ SIGNATURES = [
     ("lbm_step_timed", [_P, C.c_int32, _P], C.c_int),
     ("lbm_last_batch_time", [_P, _P, _P], C.c_int),
     ("lbm_set_kernel_timing", [_P, C.c_int32], C.c_int),
+    ("lbm_set_wide_offsets", [_P, C.c_int32], C.c_int),
     ("lbm_last_batch_kernel_time", [_P, _P, _P], C.c_int),
     ("lbm_synchronize", [_P], C.c_int),
     ("lbm_destroy", [_P], None),
@@ -353,6 +354,10 @@ class Solver:
         n = C.c_int32()
         _check(lib().lbm_last_batch_time(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def set_wide_offsets(self, on=True):
+        """Take the 64-bit-offset step path whatever the state size."""
+        _check(lib().lbm_set_wide_offsets(self._h, 1 if on else 0))
 
     def set_kernel_timing(self, on=True):
         """Time the stress-flush launches of later batches separately."""

@@ -54,6 +54,45 @@ def test_set_get_round_trip_and_init(c0_geometry):
     assert np.max(np.abs(sol.populations() - ref.f())) < 1e-13
 
 
+def _wide_vs_32bit(g, prec, steps):
+    rng = np.random.default_rng(9)
+    _, _, w = lbm.lattice(19)
+    f0 = w[None, :] * (1 + 0.05 * rng.uniform(-1, 1, (g.n, 19)))
+    out = []
+    for wide in (False, True):
+        sol = lbm.Solver(g, precision=prec, tau=TAU, dt=DT, cte_vel=U, wall_mode=lbm.WALL_HALFWAY_BB,
+                         inlet_mode=lbm.INLET_NEE_VELOCITY, outlet_mode=lbm.OUTLET_NEE_PRESSURE, parity_checks=0)
+        sol.set_wide_offsets(wide)
+        sol.set_populations(f0)
+        sol.step(steps)
+        out.append((sol.post_collision(), sol.wss()[2]))
+        sol.close()
+    return out, w
+
+
+@pytest.mark.parametrize("prec", [lbm.PREC_F32, lbm.PREC_F64])
+def test_wide_offsets_match_32bit_offsets(c0_geometry, prec):
+    # States of 4 GiB or more (the ~300M-node share) take 64-bit element
+    # offsets; on a small case that path must give the same bits.
+    out, _ = _wide_vs_32bit(c0_geometry, prec, 40)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_wide_offsets_fp16_within_one_storage_ulp(c0_geometry):
+    # FP16 storage with FP32 arithmetic: the two instantiations fuse the
+    # half-to-float conversions and multiply-adds differently, and a result
+    # within an FP32 rounding of an FP16 tie stores one FP16 ulp apart
+    # (measured: 13 of 475,551 values after one step). One step, every stored
+    # value within one FP16 ulp, at most 0.1% of them different.
+    out, w = _wide_vs_32bit(c0_geometry, lbm.PREC_F16, 1)
+    a, b = out[0][0], out[1][0]
+    g = np.abs(a - w[None, :]).astype(np.float16)
+    ulp = np.spacing(g).astype(np.float64)
+    d = np.abs(a - b)
+    assert np.all(d <= ulp * 1.0001)
+    assert np.count_nonzero(d) <= 1e-3 * d.size
+
+
 def _poiseuille(prec, steps, R=8.0, L=12, g=2e-6, tau=0.8):
     lab, origin, (ax, ay) = workloads.tube_mask(R, L)
     geom = lbm.Geometry.from_labels(lab, origin, 1.0, periodic=(0, 0, 1), normals=False)

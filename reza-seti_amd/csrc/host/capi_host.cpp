@@ -53,6 +53,7 @@ static BuildOptions to_opts(const lbm_geometry_options* o) {
   b.verify_stride = o->verify_stride;
   b.compute_normals = o->compute_normals;
   b.device = o->device;
+  b.label_device = o->label_device;
   return b;
 }
 

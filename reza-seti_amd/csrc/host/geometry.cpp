@@ -15,6 +15,8 @@
 
 #include "../lattice.h"
 #include "normals.h"
+#include "raytest.h"
+#include "voxel.h"
 
 namespace lbm {
 
@@ -47,60 +49,6 @@ void parallel_for(int64_t n, int nthreads, F&& body) {
       }
     });
   for (auto& th : pool) th.join();
-}
-
-// Geometry::isPointInsideTriangle (ComplexGeometry.cpp:208-262) for the +z
-// ray direction used by labelPoints.
-bool inside_triangle_zray(double x, double y, double z, const Triangle& T) {
-  const double x1 = T.x1, y1 = T.y1, z1 = T.z1;
-  const double x2 = T.x2, y2 = T.y2, z2 = T.z2;
-  const double x3 = T.x3, y3 = T.y3, z3 = T.z3;
-  const double dirX = 0.0, dirY = 0.0, dirZ = 1.0;
-  const double uX = x2 - x1, uY = y2 - y1, uZ = z2 - z1;
-  const double vX = x3 - x1, vY = y3 - y1, vZ = z3 - z1;
-  const double nX = uY * vZ - uZ * vY;
-  const double nY = uZ * vX - uX * vZ;
-  const double nZ = uX * vY - uY * vX;
-  const double nLengthSq = nX * nX + nY * nY + nZ * nZ;
-  const double tol = 1e-16;
-  if (nLengthSq < tol * tol) return false;
-  const double w0X = x - x1, w0Y = y - y1, w0Z = z - z1;
-  const double a = -(nX * w0X + nY * w0Y + nZ * w0Z);
-  const double b = nX * dirX + nY * dirY + nZ * dirZ;
-  if (std::fabs(b) < tol) return false;
-  const double pos = a / b;
-  const double ix = x + pos * dirX;
-  const double iy = y + pos * dirY;
-  const double iz = z + pos * dirZ;
-  const double uu = uX * uX + uY * uY + uZ * uZ;
-  const double uv = uX * vX + uY * vY + uZ * vZ;
-  const double vv = vX * vX + vY * vY + vZ * vZ;
-  const double wX = ix - x1, wY = iy - y1, wZ = iz - z1;
-  const double wu = wX * uX + wY * uY + wZ * uZ;
-  const double wv = wX * vX + wY * vY + wZ * vZ;
-  const double D = uv * uv - uu * vv;
-  const double s = (uv * wv - vv * wu) / D;
-  if (s < 0.0 || s > 1.0) return false;
-  const double t = (uv * wu - uu * wv) / D;
-  if (t < 0.0 || (s + t) > 1.0) return false;
-  return true;
-}
-
-// One triangle of Geometry::countIntersectinsTriangle (ComplexGeometry.cpp:265-310)
-// for a ray from (px,py,pz) to (ex,ey,ez).
-inline bool segment_counts(double px, double py, double pz, double ex, double ey, double ez,
-                           const Triangle& T) {
-  const double threshold = 1e-16;
-  const double dx = ex - px, dy = ey - py, dz = ez - pz;
-  const double numerator = (T.x1 - px) * T.nx + (T.y1 - py) * T.ny + (T.z1 - pz) * T.nz;
-  const double denominator = dx * T.nx + dy * T.ny + dz * T.nz;
-  if (std::fabs(denominator) < threshold) return false;
-  const double t = numerator / denominator;
-  if (!(t >= 0.0 && t <= 1.0)) return false;
-  const double ix = px + t * dx;
-  const double iy = py + t * dy;
-  const double iz = pz + t * dz;
-  return inside_triangle_zray(ix, iy, iz, T);
 }
 
 // Ray end points of labelPoints case 0 plus adjustLineCoordinates case 0
@@ -185,7 +133,7 @@ GridSpec make_grid(const BBox& bb, double grid_size) {
 }
 
 bool label_points(const GridSpec& g, const std::vector<Triangle>& tris, std::vector<uint8_t>& fluid,
-                  int nthreads, int verify_stride, std::string* err) {
+                  int nthreads, int verify_stride, std::string* err, int label_device, int device) {
   const int NX = g.NX, NY = g.NY, NZ = g.NZ;
   if (NX <= 0 || NY <= 0 || NZ <= 0) {
     if (err) *err = "empty grid";
@@ -246,6 +194,11 @@ bool label_points(const GridSpec& g, const std::vector<Triangle>& tris, std::vec
   }
 
   const double rayLength = g.bbox.max_z - g.bbox.min_z;
+  if (label_device) {
+    if (!device_label_points(tris, cand, start, rx, ry, g.bbox.min_z, g.bbox.max_z, g.spacing, NZ, fluid, device,
+                             err))
+      return false;
+  } else {
   parallel_for(ncol, nthreads, [&](int64_t c) {
     const int i = static_cast<int>(c / NY), j = static_cast<int>(c % NY);
     const double x = rx[i], y = ry[j];
@@ -262,6 +215,7 @@ bool label_points(const GridSpec& g, const std::vector<Triangle>& tris, std::vec
       col[k] = (count % 2 == 1) ? 1 : 0;
     }
   });
+  }
 
   if (verify_stride > 0) {
     std::atomic<int64_t> bad{0};
@@ -590,7 +544,7 @@ bool build_from_config(const SimulationParameters& p, const std::string& base_di
   }
   const GridSpec g = make_grid(bounding_box(tris), p.dx);
   std::vector<uint8_t> fluid;
-  if (!label_points(g, tris, fluid, o.nthreads, o.verify_stride, err)) return false;
+  if (!label_points(g, tris, fluid, o.nthreads, o.verify_stride, err, o.label_device, o.device)) return false;
   if (!build_from_labels(g, fluid, p.inlets, p.outlets, tris, o, t, err)) return false;
   if (tris_out) tris_out->swap(tris);
   return true;

@@ -47,9 +47,11 @@ GridSpec make_grid(const BBox& bb, double grid_size);
 // Ray-parity label per grid point (1 = Fluid), index GridSpec::gidx.
 // `verify_stride` > 0 re-evaluates every verify_stride-th column against all
 // triangles (no candidate filter) and returns false on any disagreement.
+// label_device = 1 evaluates the same ray test on HIP device `device`
+// (device/voxel.hip) over the same candidate lists: the same labels.
 bool label_points(const GridSpec& g, const std::vector<Triangle>& tris,
                   std::vector<uint8_t>& fluid, int nthreads, int verify_stride,
-                  std::string* err);
+                  std::string* err, int label_device = 0, int device = 0);
 
 // One (point, triangle) evaluation of the reference's ray test; exposed for
 // tests. (x, y, z) is the grid point before the bounding-box offsets.
@@ -62,6 +64,7 @@ struct BuildOptions {
   int verify_stride = 0;         // see label_points
   int compute_normals = 1;       // 0 none, 1 host threads, 2 GPU (same sum)
   int device = 0;
+  int label_device = 0;          // ray labels: 0 host threads, 1 GPU (same labels)
 };
 
 struct Tables {

@@ -48,7 +48,8 @@ class Config(C.Structure):
 
 class GeometryOptions(C.Structure):
     _fields_ = [("q", C.c_int32), ("periodic", C.c_int32 * 3), ("nthreads", C.c_int32),
-                ("verify_stride", C.c_int32), ("compute_normals", C.c_int32), ("device", C.c_int32)]
+                ("verify_stride", C.c_int32), ("compute_normals", C.c_int32), ("device", C.c_int32),
+                ("label_device", C.c_int32)]
 
 
 class GeometryInfo(C.Structure):
@@ -213,9 +214,10 @@ class Geometry:
         _check(lib().lbm_geometry_get_info(self._h, C.byref(self.info)))
 
     @staticmethod
-    def _opts(q, periodic, nthreads, verify_stride, normals):
+    def _opts(q, periodic, nthreads, verify_stride, normals, label_device=False):
         o = GeometryOptions()
         lib().lbm_geometry_options_default(C.byref(o))
+        o.label_device = int(label_device)
         o.q = q
         for a in range(3):
             o.periodic[a] = int(periodic[a])
@@ -225,8 +227,10 @@ class Geometry:
         return o
 
     @classmethod
-    def from_config(cls, path, q=19, periodic=(0, 0, 0), nthreads=0, verify_stride=0, normals=True):
-        o = cls._opts(q, periodic, nthreads, verify_stride, normals)
+    def from_config(cls, path, q=19, periodic=(0, 0, 0), nthreads=0, verify_stride=0, normals=True,
+                    label_device=False):
+        """label_device=True casts the voxeliser's rays on the GPU (same labels)."""
+        o = cls._opts(q, periodic, nthreads, verify_stride, normals, label_device)
         h = C.c_void_p()
         _check(lib().lbm_geometry_from_config(str(path).encode(), C.byref(o), C.byref(h)))
         return cls(h.value)

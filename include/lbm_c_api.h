@@ -28,7 +28,7 @@ extern "C" {
 
 #define LBM_OK 0
 #define LBM_ERR_ARG 1          /* invalid argument or inconsistent sizes */
-#define LBM_ERR_IO 2           /* config / STL could not be read */
+#define LBM_ERR_IO 2           /* config / STL could not be read, output could not be written */
 #define LBM_ERR_HIP 3          /* HIP runtime error (no device, OOM, launch) */
 #define LBM_ERR_UNSUPPORTED 4  /* mode not available for this lattice/precision */
 #define LBM_ERR_PARITY 5       /* a reference-mode assumption was violated (see lbm_step) */
@@ -244,6 +244,25 @@ int lbm_get_osi_tawss(lbm_solver* s, double* osi, double* tawss);
  * computes at the start of the next iteration; wall velocity is 0 in
  * reference wall mode). */
 int lbm_get_macroscopic(lbm_solver* s, double* rho, double* u);
+
+/* Simulation::WriteOutput (Simulation.cpp:835-856) with `dir` standing for
+ * the working directory: dir/Fluid_Variables/<iter>.vtu
+ * (OutputFile::WriteVTK_Collective, OutputFile.cpp:60-231: nodes in z-y-x
+ * order at their grid indices, Velocity / Density / VelocityMagnitude of the
+ * last collision, hexahedra over all-node cubes), dir/<iter>_FlowHemodynamics.csv
+ * (WriteSurfaceAttribute, :234-316: X,Y,Z,WSS[,OSI,TAWSS],VelMag per
+ * near-wall node) and its copy smoothed over a 0.25 radius
+ * (processFlowData / smoothData, :369-483) in
+ * dir/FlowHemodynamics/<phase_name>_<iter>_FlowHemodynamics.csv. OSI and
+ * TAWSS columns when `pulsatile`. The VTU is XML with appended raw binary
+ * data (the reference's VTK writer picks its own encoding). */
+int lbm_write_output(const lbm_geometry* g, lbm_solver* s, int32_t iter, const char* phase_name, const char* dir,
+                     int32_t pulsatile);
+/* The two host-side pieces of lbm_write_output on their own: the VTU of given
+ * node fields (rho: N, u: 3 per node, reference node order) and the
+ * smoothing of a surface CSV (radius 0.25 in the reference). */
+int lbm_write_vtu(const lbm_geometry* g, const double* rho, const double* u, const char* path);
+int lbm_smooth_surface_csv(const char* in_path, const char* out_path, double radius, int32_t pulsatile);
 
 typedef struct {
   double mass;             /* sum over nodes of rho, FP64 accumulation */

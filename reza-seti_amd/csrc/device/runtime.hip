@@ -7,7 +7,10 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -15,6 +18,7 @@
 
 #include "../capi_internal.h"
 #include "../host/checkpoint.h"
+#include "../host/output.h"
 #include "../lattice.h"
 #include "adjoint.h"
 #include "columns.h"
@@ -1632,6 +1636,58 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
 int lbm_adjoint_set_tape(lbm_solver* s, int32_t enable) {
   if (!s) return fail(LBM_ERR_ARG, "null argument");
   s->adjw.tape_on = enable != 0;
+  return LBM_OK;
+}
+
+int lbm_write_output(const lbm_geometry* g, lbm_solver* s, int32_t iter, const char* phase_name, const char* dir,
+                     int32_t pulsatile) {
+  if (!g || !s || !phase_name || !dir) return fail(LBM_ERR_ARG, "null argument");
+  const lbm::Tables& t = g->t;
+  if (t.N != s->N || t.Q != s->Q) return fail(LBM_ERR_ARG, "geometry does not match the solver");
+  const int Q = s->Q;
+  const int64_t N = s->N;
+  // Density and velocity of the last collision: the moments of its output,
+  // which BGK leaves equal to those of its input (Simulation::Collision,
+  // Simulation.cpp:77-139; wall nodes carry zero velocity in the reference
+  // wall rule).
+  std::vector<double> f(static_cast<size_t>(N) * Q);
+  int rc = lbm_get_post_collision(s, f.data());
+  if (rc) return rc;
+  const lbm::LatticeView L = lbm::lattice_view(Q);
+  std::vector<double> rho(N), u(3 * N), vm(N);
+  for (int64_t n = 0; n < N; ++n) {
+    const double* fn = f.data() + n * Q;
+    double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+    for (int q = 0; q < Q; ++q) {
+      r += fn[q];
+      jx += L.cx[q] * fn[q];
+      jy += L.cy[q] * fn[q];
+      jz += L.cz[q] * fn[q];
+    }
+    rho[n] = r;
+    const bool zero = s->prm.wall_mode == LBM_WALL_REFERENCE && t.kind[n] == LBM_KIND_WALL;
+    u[3 * n] = zero ? 0.0 : jx / r;
+    u[3 * n + 1] = zero ? 0.0 : jy / r;
+    u[3 * n + 2] = zero ? 0.0 : jz / r;
+    vm[n] = std::sqrt(u[3 * n] * u[3 * n] + u[3 * n + 1] * u[3 * n + 1] + u[3 * n + 2] * u[3 * n + 2]);
+  }
+  const size_t m = t.nearwall.size();
+  std::vector<double> wss(m), osi(pulsatile ? m : 0), tawss(pulsatile ? m : 0), nvm(m);
+  if ((rc = lbm_get_wss(s, wss.data(), nullptr, nullptr))) return rc;
+  if (pulsatile && (rc = lbm_get_osi_tawss(s, osi.data(), tawss.data()))) return rc;
+  for (size_t i = 0; i < m; ++i) nvm[i] = vm[t.nearwall[i]];
+  // Simulation::WriteOutput (Simulation.cpp:835-856), relative to `dir`.
+  const std::string base(dir), it = std::to_string(iter);
+  for (const char* sub : {"/Fluid_Variables", "/FlowHemodynamics"})
+    if (mkdir((base + sub).c_str(), 0777) != 0 && errno != EEXIST) return fail(LBM_ERR_IO, "cannot create " + base + sub);
+  std::string err;
+  const std::string raw = base + "/" + it + "_FlowHemodynamics.csv";
+  if (!lbm::write_vtu(t, rho, u, base + "/Fluid_Variables/" + it + ".vtu", &err) ||
+      !lbm::write_surface_csv(t, wss, osi, tawss, nvm, pulsatile != 0, raw, &err) ||
+      !lbm::smooth_surface_csv(raw, base + "/FlowHemodynamics/" + std::string(phase_name) + "_" + it +
+                                        "_FlowHemodynamics.csv",
+                               0.25, pulsatile != 0, &err))
+    return fail(LBM_ERR_IO, err);
   return LBM_OK;
 }
 

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "../capi_internal.h"
+#include "output.h"
 #include "../lattice.h"
 
 namespace lbm {
@@ -280,5 +281,22 @@ int lbm_geometry_get_triangles(const lbm_geometry* g, double* tris) {
 }
 
 void lbm_geometry_destroy(lbm_geometry* g) { delete g; }
+
+
+int lbm_write_vtu(const lbm_geometry* g, const double* rho, const double* u, const char* path) {
+  if (!g || !rho || !u || !path) return lbm::fail(LBM_ERR_ARG, "null argument");
+  const int64_t N = g->t.N;
+  std::vector<double> r(rho, rho + N), v(u, u + 3 * N);
+  std::string err;
+  if (!lbm::write_vtu(g->t, r, v, path, &err)) return lbm::fail(LBM_ERR_IO, err);
+  return LBM_OK;
+}
+
+int lbm_smooth_surface_csv(const char* in_path, const char* out_path, double radius, int32_t pulsatile) {
+  if (!in_path || !out_path) return lbm::fail(LBM_ERR_ARG, "null argument");
+  std::string err;
+  if (!lbm::smooth_surface_csv(in_path, out_path, radius, pulsatile != 0, &err)) return lbm::fail(LBM_ERR_IO, err);
+  return LBM_OK;
+}
 
 }  // extern "C"

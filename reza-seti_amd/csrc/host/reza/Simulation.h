@@ -189,32 +189,17 @@ class Simulation {
     iterations_done = iter;
   }
 
-  // Near-wall attributes in the column layout of
-  // OutputFile::WriteSurfaceAttribute (OutputFile.cpp:296-309): X,Y,Z,WSS
-  // [,OSI,TAWSS],VelMag; written to <phase>_<iter>_FlowHemodynamics.csv.
+  // Simulation::WriteOutput (Simulation.cpp:835-856) in output_dir: the
+  // fluid-variables VTU (Fluid_Variables/<iter>.vtu), the near-wall CSV
+  // (<iter>_FlowHemodynamics.csv: X,Y,Z,WSS[,OSI,TAWSS],VelMag) and its
+  // smoothed copy (FlowHemodynamics/<phase>_<iter>_FlowHemodynamics.csv).
   void WriteOutput(int iter, const std::string& phase_name) {
     if (output_dir.empty()) return;
-    const lbm_geometry_info& gi = SubDomain_.Geometry_.info();
-    const int64_t n = gi.n_nearwall, N = gi.n_nodes;
-    std::vector<int32_t> nodes(n), ijk(3 * N);
-    std::vector<double> nrm(3 * n), wss(n), rho(N), u(3 * N);
-    check(lbm_geometry_get_nearwall(SubDomain_.Geometry_.handle(), nodes.data(), nrm.data()), "nearwall");
-    check(lbm_geometry_get_nodes(SubDomain_.Geometry_.handle(), ijk.data(), nullptr, nullptr), "nodes");
-    check(lbm_get_wss(solver_.get(), wss.data(), nullptr, nullptr), "wss");
-    check(lbm_get_macroscopic(solver_.get(), rho.data(), u.data()), "macroscopic");
     const bool pulsatile = simParams.FlowType == "Pulsatile";
     if (pulsatile) ComputeOSIAndTAWSS(iter);
-    std::ofstream f(output_dir + "/" + phase_name + "_" + std::to_string(iter) + "_FlowHemodynamics.csv");
-    f << (pulsatile ? "X,Y,Z,WSS,OSI,TAWSS,VelMag\n" : "X,Y,Z,WSS,VelMag\n");
-    for (int64_t i = 0; i < n; ++i) {
-      const int32_t m = nodes[i];
-      const double x = gi.bbox[0] + ijk[3 * m] * gi.spacing, y = gi.bbox[2] + ijk[3 * m + 1] * gi.spacing,
-                   z = gi.bbox[4] + ijk[3 * m + 2] * gi.spacing;
-      const double vm = std::sqrt(u[3 * m] * u[3 * m] + u[3 * m + 1] * u[3 * m + 1] + u[3 * m + 2] * u[3 * m + 2]);
-      f << x << "," << y << "," << z << "," << wss[i];
-      if (pulsatile) f << "," << osi[i] << "," << tawss[i];
-      f << "," << vm << "\n";
-    }
+    check(lbm_write_output(SubDomain_.Geometry_.handle(), solver_.get(), iter, phase_name.c_str(),
+                           output_dir.c_str(), pulsatile ? 1 : 0),
+          "WriteOutput");
   }
 
   lbm_solver* solver() const { return solver_.get(); }

@@ -122,6 +122,9 @@ SIGNATURES = [
     ("lbm_last_batch_time", [_P, _P, _P], C.c_int),
     ("lbm_set_kernel_timing", [_P, C.c_int32], C.c_int),
     ("lbm_set_wide_offsets", [_P, C.c_int32], C.c_int),
+    ("lbm_write_output", [_P, _P, C.c_int32, C.c_char_p, C.c_char_p, C.c_int32], C.c_int),
+    ("lbm_write_vtu", [_P, _P, _P, C.c_char_p], C.c_int),
+    ("lbm_smooth_surface_csv", [C.c_char_p, C.c_char_p, C.c_double, C.c_int32], C.c_int),
     ("lbm_last_batch_kernel_time", [_P, _P, _P], C.c_int),
     ("lbm_synchronize", [_P], C.c_int),
     ("lbm_destroy", [_P], None),
@@ -188,6 +191,12 @@ def parse_config(path):
     cfg = Config()
     _check(lib().lbm_config_parse(str(path).encode(), C.byref(cfg)))
     return cfg
+
+
+def smooth_surface_csv(in_path, out_path, radius=0.25, pulsatile=False):
+    """OutputFile::processFlowData's smoothing of a near-wall CSV."""
+    _check(lib().lbm_smooth_surface_csv(str(in_path).encode(), str(out_path).encode(), float(radius),
+                                        1 if pulsatile else 0))
 
 
 def plane_block(plane, coordinate, normal, range_x=None, range_y=None, range_z=None):
@@ -261,6 +270,13 @@ class Geometry:
     @property
     def q(self):
         return self.info.q
+
+    def write_vtu(self, rho, u, path):
+        """OutputFile::WriteVTK_Collective's VTU of node fields (reference node order)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert rho.shape == (self.n,) and u.shape == (self.n, 3)
+        _check(lib().lbm_write_vtu(self._h, _ptr(rho), _ptr(u), str(path).encode()))
 
     def labels(self):
         g = tuple(self.info.grid)
@@ -358,6 +374,11 @@ class Solver:
         n = C.c_int32()
         _check(lib().lbm_last_batch_time(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def write_output(self, geom, iteration, phase_name, directory, pulsatile=False):
+        """Simulation::WriteOutput: the VTU, the near-wall CSV and its smoothed copy."""
+        _check(lib().lbm_write_output(geom._h, self._h, int(iteration), phase_name.encode(), str(directory).encode(),
+                                      1 if pulsatile else 0))
 
     def set_wide_offsets(self, on=True):
         """Take the 64-bit-offset step path whatever the state size."""

@@ -40,6 +40,9 @@ def test_driver_runs_the_reference_loop_bit_exact(tmp_path):
     ref = oracle.Reference.from_geometry(geom, tau=0.8, dt=1e-5)
     ref.step(np.full(150, 0.03))
     assert np.array_equal(f, ref.f())
-    csv = out / "Iteration_100_100_FlowHemodynamics.csv"
-    lines = csv.read_text().splitlines()
-    assert lines[0] == "X,Y,Z,WSS,VelMag" and len(lines) == geom.info.n_nearwall + 1
+    # Simulation::WriteOutput's three files (Simulation.cpp:835-856).
+    raw = (out / "100_FlowHemodynamics.csv").read_text().splitlines()
+    assert raw[0] == "X,Y,Z,WSS,VelMag" and len(raw) == geom.info.n_nearwall + 1
+    smooth = (out / "FlowHemodynamics" / "Iteration_100_100_FlowHemodynamics.csv").read_text().splitlines()
+    assert smooth == raw  # pitch 1 > the 0.25 smoothing radius: no neighbours, values kept
+    assert (out / "Fluid_Variables" / "100.vtu").stat().st_size > 0

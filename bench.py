@@ -58,7 +58,7 @@ def coefficients(t0, n):
 
 def build_geometry(workdir):
     t0 = time.time()
-    cfg, info = workloads.c1_bifurcation(workdir)
+    cfg, info = workloads.y_bifurcation(workdir)
     t1 = time.time()
     geom = lbm.Geometry.from_config(cfg, normals=2)
     t2 = time.time()
@@ -200,16 +200,16 @@ def run_adjoint(geom, info, steps=ADJ_STEPS):
     }
 
 
-C2_TAU = 0.55
+STENOSIS_TAU = 0.55
 
 
-def run_c2(steps, warmup=20):
+def run_stenosis(steps, warmup=20):
     """D3Q27 stenosed tube (radius 64, length 1024, 50% area cosine
     stenosis at z = 384 over 128 lu, ~12.7M nodes), parabolic NEE inlet at
     Re 200 on the diameter, NEE zero-pressure outlet, bounce-back walls;
     FP32-shifted and FP64 step rates with the stress computed every step."""
     with tempfile.TemporaryDirectory() as wd:
-        cfg, info = workloads.c2_stenosis(wd, tau=C2_TAU)
+        cfg, info = workloads.stenosed_tube(wd, tau=STENOSIS_TAU)
         geom = lbm.Geometry.from_config(cfg, q=27, normals=2)
     ijk, kind, _ = geom.nodes()
     b = geom.info.bbox
@@ -219,13 +219,13 @@ def run_c2(steps, warmup=20):
     y = b[2] + ijk[inl, 1] * h - info["axis"][1]
     prof = np.clip(1.0 - (x * x + y * y) / info["radius"] ** 2, 0.0, None)
     prof /= prof.mean()
-    nu = (C2_TAU - 0.5) / 3.0
+    nu = (STENOSIS_TAU - 0.5) / 3.0
     u_mean = 200.0 * nu / (2.0 * info["radius"])
     out = {"workload": f"D3Q27 stenosed tube, {geom.n} nodes, parabolic inlet Re 200 (U_mean {u_mean:.4f}), "
-                       f"tau {C2_TAU}, bounce-back walls, NEE inlet/outlet", "nodes": geom.n, "steps": steps}
+                       f"tau {STENOSIS_TAU}, bounce-back walls, NEE inlet/outlet", "nodes": geom.n, "steps": steps}
     for name, prec, nbytes in (("f32", lbm.PREC_F32, 2 * 27 * 4), ("f64", lbm.PREC_F64, 2 * 27 * 8)):
         s = lbm.Solver(geom, precision=prec, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NEE_VELOCITY,
-                       outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=C2_TAU, dt=1e-5, parity_checks=0, wss_every=1)
+                       outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=STENOSIS_TAU, dt=1e-5, parity_checks=0, wss_every=1)
         s.set_inlet_profile(prof[None, :])
         s.step_coefficients(np.full((warmup, 1), u_mean))
         s.synchronize()
@@ -241,10 +241,10 @@ def run_c2(steps, warmup=20):
     return out
 
 
-C4_SCALE, C4_WO, C4_PERIOD, C4_WSS_EVERY = 4.3, 5.0, 8000, 100
+LARGE_SCALE, LARGE_WO, LARGE_PERIOD, LARGE_WSS_EVERY = 4.3, 5.0, 8000, 100
 
 
-def run_c4(steps, warmup=20):
+def run_large_share(steps, warmup=20):
     """Large single-GPU share: the tortuous vessel (Catmull-Rom centreline
     through a seeded 12-point random walk, radius 20-40 and a saccular bulge
     of radius 60, all lengths x4.3: ~300M nodes), D3Q19 FP32-shifted,
@@ -252,7 +252,7 @@ def run_c4(steps, warmup=20):
     outlet, bounce-back walls, wall shear stress every 100 steps. Returns the
     step rate and its roofline fraction, with the set-up times."""
     t0 = time.perf_counter()
-    fluid, origin, info = workloads.c4_vessel(scale=C4_SCALE)
+    fluid, origin, info = workloads.tortuous_vessel(scale=LARGE_SCALE)
     t1 = time.perf_counter()
     print(f"# large share: labels {fluid.shape} in {t1 - t0:.1f}s", file=sys.stderr, flush=True)
     nz = fluid.shape[2]
@@ -269,16 +269,16 @@ def run_c4(steps, warmup=20):
     c0, r0 = info["centreline"][0], float(info["radius"][0])
     r = np.hypot(origin[0] + ijk[ii, 0] - c0[0], origin[1] + ijk[ii, 1] - c0[1]) / r0
     del ijk, kind
-    basis = womersley_basis(r, C4_WO)
-    w = 2.0 * math.pi / C4_PERIOD
-    tau = 3.0 * r0 * r0 * w / (C4_WO * C4_WO) + 0.5
+    basis = womersley_basis(r, LARGE_WO)
+    w = 2.0 * math.pi / LARGE_PERIOD
+    tau = 3.0 * r0 * r0 * w / (LARGE_WO * LARGE_WO) + 0.5
 
     def coef(t0_, n):
         t = np.arange(t0_, t0_ + n, dtype=np.float64)
         return np.stack([np.full(n, U_MEAN), U_AMP * np.cos(w * t), U_AMP * np.sin(w * t)], 1)
 
     s = lbm.Solver(geom, precision=lbm.PREC_F32, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NEE_VELOCITY,
-                   outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=tau, dt=1e-5, parity_checks=0, wss_every=C4_WSS_EVERY)
+                   outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=tau, dt=1e-5, parity_checks=0, wss_every=LARGE_WSS_EVERY)
     s.set_inlet_profile(basis)
     t3 = time.perf_counter()
     print(f"# large share: solver {t3 - t2:.1f}s", file=sys.stderr, flush=True)
@@ -291,9 +291,9 @@ def run_c4(steps, warmup=20):
     ev_ms, _ = s.last_batch_ms()
     n = geom.n
     achieved = n * 152.0 / (ev_ms / 1e3 / steps) / 1e9
-    res = {"workload": f"tortuous vessel x{C4_SCALE} (12-point walk, radius {20 * C4_SCALE:.0f}-{40 * C4_SCALE:.0f}, "
-                       f"bulge {60 * C4_SCALE:.0f}), D3Q19 FP32-shifted, Womersley Wo {C4_WO} period {C4_PERIOD}, "
-                       f"WSS every {C4_WSS_EVERY} steps",
+    res = {"workload": f"tortuous vessel x{LARGE_SCALE} (12-point walk, radius {20 * LARGE_SCALE:.0f}-{40 * LARGE_SCALE:.0f}, "
+                       f"bulge {60 * LARGE_SCALE:.0f}), D3Q19 FP32-shifted, Womersley Wo {LARGE_WO} period {LARGE_PERIOD}, "
+                       f"WSS every {LARGE_WSS_EVERY} steps",
            "nodes": n, "grid": grid, "triangles": info["n_triangles"], "steps": steps, "tau": round(tau, 4),
            "f32_mlups": round(n * steps / wall / 1e6, 1), "roofline_frac": round(achieved / PEAK_HBM_GBS, 4),
            "achieved_gbs": round(achieved, 1), "launch_us": round(ev_ms * 1e3 / steps, 2),
@@ -323,7 +323,7 @@ def cpu_baseline(geom, budget_s=12.0):
                       f"reference step (bounce-back walls, NEE inlet/outlet), OpenMP on {ncores} host threads"}
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "columns_step_c1_bifurcation_f32_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "columns_step_bifurcation_f32_traffic.md")
 
 
 def profiled_traffic():
@@ -353,14 +353,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="f32,f64,f16")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-c2", action="store_true")
-    ap.add_argument("--c2-steps", type=int, default=500)
+    ap.add_argument("--no-stenosis", action="store_true")
+    ap.add_argument("--stenosis-steps", type=int, default=500)
     ap.add_argument("--adjoint-steps", type=int, default=ADJ_STEPS)
-    ap.add_argument("--c4", action="store_true", help="also run the ~300M-node large share (minutes of set-up)")
-    ap.add_argument("--c4-steps", type=int, default=300)
+    ap.add_argument("--large-share", action="store_true", help="also run the ~300M-node large share (minutes of set-up)")
+    ap.add_argument("--large-share-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the whole lattice on a single MI355X "
+        print("bench.py measures one GPU: the whole lattice runs on a single MI355X "
               "(no sharding or collectives); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
@@ -390,7 +390,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(head["wall"] * 1e3 / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "C1 precision-study Y-bifurcation, D3Q19, FP32-shifted populations",
+        "config": {"workload": "precision-study Y-bifurcation (parent radius 24, daughters 18 at 35 degrees), "
+                                "2.85M nodes, D3Q19, FP32-shifted populations",
                    "nodes": N, "explicit_row_nodes": int(sinfo.n_special), "triangles": info["n_triangles"],
                    "inlet": f"Womersley Wo={WO} period {PERIOD} steps, U_mean {U_MEAN}",
                    "outlet": "NEE pressure rho=1", "walls": "half-way bounce-back", "tau": round(tau_for(), 6)},
@@ -425,12 +426,12 @@ def main():
               f"reverse {adj['backward_s']:.3f}s", file=sys.stderr, flush=True)
         out["adjoint_over_forward"] = adj["adjoint_over_forward"]
         out["adjoint"] = adj
-    if not args.no_c2:
-        out["d3q27_stenosis"] = run_c2(args.c2_steps)
+    if not args.no_stenosis:
+        out["d3q27_stenosis"] = run_stenosis(args.stenosis_steps)
         print(f"# D3Q27 stenosis: f32 {out['d3q27_stenosis']['f32_mlups']:.0f} MLUPS, "
               f"f64 {out['d3q27_stenosis']['f64_mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
-    if args.c4:
-        out["large_share"] = run_c4(args.c4_steps)
+    if args.large_share:
+        out["large_share"] = run_large_share(args.large_share_steps)
         print(f"# large share: {out['large_share']['nodes']} nodes, {out['large_share']['f32_mlups']:.0f} MLUPS, "
               f"roofline {out['large_share']['roofline_frac']:.3f}", file=sys.stderr, flush=True)
     if not args.no_cpu:

@@ -97,7 +97,7 @@ def write_config(path, stl_file, *, tau, cte_vel=0.05, max_iter=2000, dt=1e-5, g
         f.write("\n".join(text) + "\n")
 
 
-def c0_tube(workdir, radius=10.0, length=80.0, tau=0.8, cte_vel=0.05, max_iter=2000, segments=64):
+def straight_tube(workdir, radius=10.0, length=80.0, tau=0.8, cte_vel=0.05, max_iter=2000, segments=64):
     """Straight tube: 64-segment cylinder, radius 10,
     length 80, inlet cap at z = 0 (normal -z, reference Zou-He), outlet cap at
     z = length (normal +z, reference copy rule). Returns the config path."""
@@ -153,7 +153,7 @@ def stenosis_radius(radius=64.0, z_center=384.0, length=128.0, area_reduction=0.
     return r
 
 
-def c2_stenosis(workdir, radius=64.0, length=1024.0, z_center=384.0, sten_length=128.0, area_reduction=0.5,
+def stenosed_tube(workdir, radius=64.0, length=1024.0, z_center=384.0, sten_length=128.0, area_reduction=0.5,
                 segments=256, tau=0.55, max_iter=20000):
     """Stenosed tube: radius 64, length 1024, cosine stenosis of 50% area
     reduction centred at z = 384 over 128 lu (a 130 x 130 x 1025 grid at
@@ -389,7 +389,7 @@ def bifurcation_function(parent_radius=24.0, daughter_radius=18.0, half_angle_de
                            daughter_radius=daughter_radius)
 
 
-def c1_bifurcation(workdir, mesh_h=3.5, **kw):
+def y_bifurcation(workdir, mesh_h=3.5, **kw):
     """Precision-study Y-bifurcation (parent r 24, daughters r 18 at 35
     degrees, 3-mode wall perturbation of amplitude 2, seed 1). Inlet: the z = 0
     cap (normal -z); outlet: the top cut plane (normal +z), set on the last
@@ -458,7 +458,7 @@ def _catmull_rom(P, samples_per_seg):
     return np.vstack(out)
 
 
-def c4_vessel(scale=4.5, seed=5, device=None, sample_step=2.0, slab=32):
+def tortuous_vessel(scale=4.5, seed=5, device=None, sample_step=2.0, slab=32):
     """Tortuous vessel of the large single-GPU share: a Catmull-Rom
     centreline through a 12-point random walk (step 80 scale lu, a persistent
     walk heading +z whose first and last steps run along z), radius varying

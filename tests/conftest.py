@@ -14,15 +14,15 @@ def pytest_configure(config):
 
 
 @pytest.fixture(scope="session")
-def c0_config(tmp_path_factory):
+def tube_config(tmp_path_factory):
     import workloads
 
     d = tmp_path_factory.mktemp("c0")
-    return workloads.c0_tube(str(d))
+    return workloads.straight_tube(str(d))
 
 
 @pytest.fixture(scope="session")
-def c0_geometry(c0_config):
+def tube_geometry(tube_config):
     import lbm
 
-    return lbm.Geometry.from_config(c0_config)
+    return lbm.Geometry.from_config(tube_config)

@@ -24,7 +24,7 @@ def test_driver_reports_a_missing_config():
 def test_driver_runs_the_reference_loop_bit_exact(tmp_path):
     import lbm
 
-    cfg = workloads.c0_tube(str(tmp_path), radius=6.0, length=30.0, cte_vel=0.03, max_iter=150)
+    cfg = workloads.straight_tube(str(tmp_path), radius=6.0, length=30.0, cte_vel=0.03, max_iter=150)
     dump = tmp_path / "f.bin"
     out = tmp_path / "out"
     out.mkdir()

@@ -122,9 +122,9 @@ def test_periodic_labels_and_wrap():
         assert m >= 0 and ijk[m, 2] == 0 and (ijk[m, :2] == ijk[n, :2]).all()
 
 
-def test_bc_neighbour_points_behind_the_plane(c0_geometry):
-    ijk, kind, _ = c0_geometry.nodes()
-    bcn = c0_geometry.bc_neighbour()
+def test_bc_neighbour_points_behind_the_plane(tube_geometry):
+    ijk, kind, _ = tube_geometry.nodes()
+    bcn = tube_geometry.bc_neighbour()
     for n in np.where(kind == lbm.KIND_OUTLET)[0]:
         m = bcn[n]
         assert m >= 0 and (ijk[m] == ijk[n] - [0, 0, 1]).all()
@@ -134,8 +134,8 @@ def test_bc_neighbour_points_behind_the_plane(c0_geometry):
     assert (bcn[kind <= lbm.KIND_WALL] == -1).all()
 
 
-def test_c0_counts(c0_geometry):
-    i = c0_geometry.info
+def test_tube_counts(tube_geometry):
+    i = tube_geometry.info
     # ~25k nodes for the radius-10, length-80 tube; every node is classified
     assert 24000 < i.n_nodes < 26000
     assert i.n_fluid + i.n_wall + i.n_inlet + i.n_outlet == i.n_nodes
@@ -143,10 +143,10 @@ def test_c0_counts(c0_geometry):
     assert tuple(i.grid) == (21, 21, 81) and i.spacing == 1.0
 
 
-def test_cylinder_normals_are_radial_mid_tube(c0_geometry):
-    ijk, _, _ = c0_geometry.nodes()
-    nw, nrm = c0_geometry.nearwall()
-    b = c0_geometry.info.bbox
+def test_cylinder_normals_are_radial_mid_tube(tube_geometry):
+    ijk, _, _ = tube_geometry.nodes()
+    nw, nrm = tube_geometry.nearwall()
+    b = tube_geometry.info.bbox
     mid = np.abs(ijk[nw, 2] - 40) <= 2
     x = b[0] + ijk[nw, 0] - 0.37
     y = b[2] + ijk[nw, 1] - 0.21

@@ -23,7 +23,7 @@ M = 3
 @pytest.fixture(scope="module")
 def setup(tmp_path_factory):
     wd = str(tmp_path_factory.mktemp("adj"))
-    cfg = workloads.c0_tube(wd, radius=5.0, length=20.0, tau=TAU, cte_vel=0.03, max_iter=T)
+    cfg = workloads.straight_tube(wd, radius=5.0, length=20.0, tau=TAU, cte_vel=0.03, max_iter=T)
     geom = lbm.Geometry.from_config(cfg)
     ijk, kind, _ = geom.nodes()
     inl = np.where(kind == lbm.KIND_INLET)[0]

@@ -20,8 +20,8 @@ TAU, DT, U = 0.8, 1e-5, 0.04
 
 @pytest.mark.parametrize("inlet,outlet", [(lbm.INLET_NEE_VELOCITY, lbm.OUTLET_NEE_PRESSURE),
                                           (lbm.INLET_REFERENCE_ZOUHE, lbm.OUTLET_REFERENCE)])
-def test_bounce_back_family_matches_oracle(c0_geometry, inlet, outlet):
-    g = c0_geometry
+def test_bounce_back_family_matches_oracle(tube_geometry, inlet, outlet):
+    g = tube_geometry
     sol = lbm.Solver(g, tau=TAU, dt=DT, cte_vel=U, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=inlet,
                      outlet_mode=outlet, parity_checks=0)
     ref = oracle.Reference.from_geometry(g, tau=TAU, dt=DT, wall_mode=1, inlet_mode=inlet, outlet_mode=outlet)
@@ -39,15 +39,15 @@ def test_bounce_back_family_matches_oracle(c0_geometry, inlet, outlet):
     assert np.max(np.abs(mag - rmag)) <= 1e-9 * np.max(np.abs(rmag))
 
 
-def test_set_get_round_trip_and_init(c0_geometry):
+def test_set_get_round_trip_and_init(tube_geometry):
     rng = np.random.default_rng(5)
     _, _, w = lbm.lattice(19)
-    f0 = w[None, :] * (1 + 0.05 * rng.uniform(-1, 1, (c0_geometry.n, 19)))
-    sol = lbm.Solver(c0_geometry, tau=TAU, dt=DT, cte_vel=U, wall_mode=lbm.WALL_HALFWAY_BB,
+    f0 = w[None, :] * (1 + 0.05 * rng.uniform(-1, 1, (tube_geometry.n, 19)))
+    sol = lbm.Solver(tube_geometry, tau=TAU, dt=DT, cte_vel=U, wall_mode=lbm.WALL_HALFWAY_BB,
                      inlet_mode=lbm.INLET_NEE_VELOCITY, outlet_mode=lbm.OUTLET_NEE_PRESSURE, parity_checks=0)
     sol.set_populations(f0)
     assert np.array_equal(sol.populations(), f0)
-    ref = oracle.Reference.from_geometry(c0_geometry, tau=TAU, dt=DT, wall_mode=1, inlet_mode=1, outlet_mode=1)
+    ref = oracle.Reference.from_geometry(tube_geometry, tau=TAU, dt=DT, wall_mode=1, inlet_mode=1, outlet_mode=1)
     ref.set_f(f0)
     sol.step(3)
     ref.step(np.full(3, U))
@@ -71,20 +71,20 @@ def _wide_vs_32bit(g, prec, steps):
 
 
 @pytest.mark.parametrize("prec", [lbm.PREC_F32, lbm.PREC_F64])
-def test_wide_offsets_match_32bit_offsets(c0_geometry, prec):
+def test_wide_offsets_match_32bit_offsets(tube_geometry, prec):
     # States of 4 GiB or more (the ~300M-node share) take 64-bit element
     # offsets; on a small case that path must give the same bits.
-    out, _ = _wide_vs_32bit(c0_geometry, prec, 40)
+    out, _ = _wide_vs_32bit(tube_geometry, prec, 40)
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
-def test_wide_offsets_fp16_within_one_storage_ulp(c0_geometry):
+def test_wide_offsets_fp16_within_one_storage_ulp(tube_geometry):
     # FP16 storage with FP32 arithmetic: the two instantiations fuse the
     # half-to-float conversions and multiply-adds differently, and a result
     # within an FP32 rounding of an FP16 tie stores one FP16 ulp apart
     # (measured: 13 of 475,551 values after one step). One step, every stored
     # value within one FP16 ulp, at most 0.1% of them different.
-    out, w = _wide_vs_32bit(c0_geometry, lbm.PREC_F16, 1)
+    out, w = _wide_vs_32bit(tube_geometry, lbm.PREC_F16, 1)
     a, b = out[0][0], out[1][0]
     g = np.abs(a - w[None, :]).astype(np.float16)
     ulp = np.spacing(g).astype(np.float64)

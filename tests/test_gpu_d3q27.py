@@ -22,7 +22,7 @@ TAU, DT, U = 0.7, 1e-5, 0.03
 @pytest.fixture(scope="module")
 def sten(tmp_path_factory):
     wd = str(tmp_path_factory.mktemp("sten"))
-    cfg, _ = workloads.c2_stenosis(wd, radius=8.0, length=40.0, z_center=20.0, sten_length=16.0, segments=64,
+    cfg, _ = workloads.stenosed_tube(wd, radius=8.0, length=40.0, z_center=20.0, sten_length=16.0, segments=64,
                                    tau=TAU)
     return lbm.Geometry.from_config(cfg, q=27)
 

@@ -40,7 +40,7 @@ def test_bifurcation_fp32_tracks_fp64():
 
 def test_stenosis_d3q27_fp32_tracks_fp64():
     with tempfile.TemporaryDirectory() as wd:
-        cfg, info = workloads.c2_stenosis(wd, tau=bench.C2_TAU)
+        cfg, info = workloads.stenosed_tube(wd, tau=bench.STENOSIS_TAU)
         geom = lbm.Geometry.from_config(cfg, q=27, normals=2, label_device=True)
     ijk, kind, _ = geom.nodes()
     b, h = geom.info.bbox, geom.info.spacing
@@ -49,11 +49,11 @@ def test_stenosis_d3q27_fp32_tracks_fp64():
     y = b[2] + ijk[inl, 1] * h - info["axis"][1]
     prof = np.clip(1.0 - (x * x + y * y) / info["radius"] ** 2, 0.0, None)
     prof /= prof.mean()
-    u_mean = 200.0 * (bench.C2_TAU - 0.5) / 3.0 / (2.0 * info["radius"])
+    u_mean = 200.0 * (bench.STENOSIS_TAU - 0.5) / 3.0 / (2.0 * info["radius"])
     out = {}
     for prec in (lbm.PREC_F64, lbm.PREC_F32):
         s = lbm.Solver(geom, precision=prec, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NEE_VELOCITY,
-                       outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=bench.C2_TAU, dt=1e-5, parity_checks=0, wss_every=1)
+                       outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=bench.STENOSIS_TAU, dt=1e-5, parity_checks=0, wss_every=1)
         s.set_inlet_profile(prof[None, :])
         s.step_coefficients(np.full((300, 1), u_mean))
         out[prec] = (s.wss()[0], s.macroscopic()[1])

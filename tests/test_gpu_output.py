@@ -11,8 +11,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("wall_mode", [lbm.WALL_REFERENCE, lbm.WALL_HALFWAY_BB])
-def test_write_output_files(tmp_path, c0_geometry, wall_mode):
-    g = c0_geometry
+def test_write_output_files(tmp_path, tube_geometry, wall_mode):
+    g = tube_geometry
     kw = {} if wall_mode == lbm.WALL_REFERENCE else dict(inlet_mode=lbm.INLET_NEE_VELOCITY,
                                                          outlet_mode=lbm.OUTLET_NEE_PRESSURE)
     sol = lbm.Solver(g, tau=0.8, dt=1e-5, cte_vel=0.04, wall_mode=wall_mode, parity_checks=0, **kw)

@@ -28,8 +28,8 @@ def _assert_same(a, b, what):
     assert d == 0.0, f"{what}: max |diff| = {d:.3e}"
 
 
-def test_c0_tube_2000_steps_bit_exact(c0_geometry):
-    sol, ref = _pair(c0_geometry)
+def test_tube_2000_steps_bit_exact(tube_geometry):
+    sol, ref = _pair(tube_geometry)
     done = 0
     for target in (1, 10, 100, 2000):
         sol.step(target - done)
@@ -52,11 +52,11 @@ def _collided(ref):
     return ref.fs()
 
 
-def test_random_populations_bit_exact(c0_geometry):
+def test_random_populations_bit_exact(tube_geometry):
     rng = np.random.default_rng(12345)
     _, _, w = lbm.lattice(19)
-    f0 = w[None, :] * (1.0 + 0.05 * rng.uniform(-1, 1, size=(c0_geometry.n, 19)))
-    sol, ref = _pair(c0_geometry)
+    f0 = w[None, :] * (1.0 + 0.05 * rng.uniform(-1, 1, size=(tube_geometry.n, 19)))
+    sol, ref = _pair(tube_geometry)
     sol.set_populations(f0)
     ref.set_f(f0)
     _assert_same(sol.populations(), f0, "set/get round trip")
@@ -66,7 +66,7 @@ def test_random_populations_bit_exact(c0_geometry):
 
 
 def test_pulsatile_inlet_bit_exact(tmp_path):
-    cfg = workloads.c0_tube(str(tmp_path), radius=6.0, length=30.0)
+    cfg = workloads.straight_tube(str(tmp_path), radius=6.0, length=30.0)
     geom = lbm.Geometry.from_config(cfg)
     sol = lbm.Solver(geom, tau=TAU, dt=DT, flow_type=lbm.FLOW_PULSATILE)
     ref = oracle.Reference.from_geometry(geom, tau=TAU, dt=DT)
@@ -79,8 +79,8 @@ def test_pulsatile_inlet_bit_exact(tmp_path):
     _assert_same(sol.populations(), ref.f(), "f after 50 pulsatile steps")
 
 
-def test_reduction_kernel_matches_host_sums(c0_geometry):
-    sol = lbm.Solver(c0_geometry, tau=TAU, dt=DT, cte_vel=U)
+def test_reduction_kernel_matches_host_sums(tube_geometry):
+    sol = lbm.Solver(tube_geometry, tau=TAU, dt=DT, cte_vel=U)
     sol.step(50)
     s = sol.reduce()
     fs = sol.post_collision()
@@ -96,9 +96,9 @@ def test_reduction_kernel_matches_host_sums(c0_geometry):
 
 
 @pytest.mark.parametrize("prec,tol", [(lbm.PREC_F32, 2e-5), (lbm.PREC_F16, 2e-2)])
-def test_reduced_precision_tracks_fp64(c0_geometry, prec, tol):
-    a = lbm.Solver(c0_geometry, tau=TAU, dt=DT, cte_vel=U)
-    b = lbm.Solver(c0_geometry, tau=TAU, dt=DT, cte_vel=U, precision=prec, parity_checks=0)
+def test_reduced_precision_tracks_fp64(tube_geometry, prec, tol):
+    a = lbm.Solver(tube_geometry, tau=TAU, dt=DT, cte_vel=U)
+    b = lbm.Solver(tube_geometry, tau=TAU, dt=DT, cte_vel=U, precision=prec, parity_checks=0)
     a.step(200)
     b.step(200)
     fa, fb = a.populations(), b.populations()

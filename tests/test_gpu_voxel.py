@@ -34,17 +34,17 @@ def _same(host, dev):
     assert np.array_equal(host.neighbours(), dev.neighbours())
 
 
-def test_tube_labels_equal_host(c0_config):
-    _same(*_both(c0_config))
+def test_tube_labels_equal_host(tube_config):
+    _same(*_both(tube_config))
 
 
 def test_stenosis_labels_equal_host(tmp_path):
-    cfg, _ = workloads.c2_stenosis(str(tmp_path), radius=24.0, length=256.0, z_center=96.0, sten_length=48.0)
+    cfg, _ = workloads.stenosed_tube(str(tmp_path), radius=24.0, length=256.0, z_center=96.0, sten_length=48.0)
     _same(*_both(cfg, q=27))
 
 
 def test_bifurcation_labels_equal_host(tmp_path):
-    cfg, _ = workloads.c1_bifurcation(str(tmp_path), mesh_h=4.0, parent_radius=12.0, daughter_radius=9.0,
+    cfg, _ = workloads.y_bifurcation(str(tmp_path), mesh_h=4.0, parent_radius=12.0, daughter_radius=9.0,
                                       parent_length=120.0, daughter_length=160.0)
     host, dev = _both(cfg)
     _same(host, dev)

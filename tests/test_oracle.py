@@ -161,12 +161,12 @@ def test_inlet_waveform_spot_values():
     assert abs(oracle.inlet_velocity(x + p, p) - eq3) < 1e-9         # periodic in t
 
 
-def test_oracle_runs_c0_without_nan(c0_geometry):
-    ref = oracle.Reference.from_geometry(c0_geometry, tau=0.8, dt=1e-5)
+def test_oracle_runs_the_tube_without_nan(tube_geometry):
+    ref = oracle.Reference.from_geometry(tube_geometry, tau=0.8, dt=1e-5)
     ref.step(np.full(50, 0.05), nthreads=4)
     f = ref.f()
     assert np.isfinite(f).all()
-    _, kind, _ = c0_geometry.nodes()
+    _, kind, _ = tube_geometry.nodes()
     rho = f.sum(1)
     assert 0.9 < rho.min() and rho.max() < 1.5
     # the inlet nodes carry the imposed speed after Zou-He

@@ -27,8 +27,8 @@ def read_vtu(path):
     return int(piece.group(1)), int(piece.group(2)), arrays
 
 
-def test_vtu_matches_the_reference_writer_loops(tmp_path, c0_geometry):
-    g = c0_geometry
+def test_vtu_matches_the_reference_writer_loops(tmp_path, tube_geometry):
+    g = tube_geometry
     rng = np.random.default_rng(2)
     rho = 1.0 + 0.01 * rng.standard_normal(g.n)
     u = 0.01 * rng.standard_normal((g.n, 3))

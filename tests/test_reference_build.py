@@ -58,7 +58,7 @@ def test_lattice_tables_equal_reference():
 
 
 def test_tube_tables_equal_reference(tmp_path):
-    cfg = workloads.c0_tube(str(tmp_path), radius=6.0, length=24.0)
+    cfg = workloads.straight_tube(str(tmp_path), radius=6.0, length=24.0)
     _compare_tables(lbm.Geometry.from_config(cfg), refbind.build(cfg))
 
 

@@ -11,7 +11,7 @@ import workloads
 
 @pytest.fixture(scope="module")
 def vessel():
-    return workloads.c4_vessel(scale=0.5, device="cpu")
+    return workloads.tortuous_vessel(scale=0.5, device="cpu")
 
 
 def test_vessel_volume_matches_the_swept_tube(vessel):

@@ -521,9 +521,7 @@ def tortuous_vessel(scale=4.5, seed=5, device=None, sample_step=2.0, slab=32):
         k1 = min(nz, k0 + slab)
         za, zb = origin[2] + k0, origin[2] + k1 - 1
         sel = np.where((C[:, 2] + R >= za) & (C[:, 2] - R <= zb))[0]
-        boxes = []
         if sel.size:
-            boxes.append((C[sel, :2] - R[sel, None]).min(0), )
             blo = (C[sel, :2] - R[sel, None]).min(0)
             bhi = (C[sel, :2] + R[sel, None]).max(0)
         else:
@@ -543,9 +541,12 @@ def tortuous_vessel(scale=4.5, seed=5, device=None, sample_step=2.0, slab=32):
         inside = ((pts - cbt) ** 2).sum(1) < rb * rb
         if sel.size:
             cs, rs = Ct[torch.as_tensor(sel, device=dev)], Rt[torch.as_tensor(sel, device=dev)]
-            for a in range(0, pts.shape[0], 1 << 20):
-                d2 = torch.cdist(pts[a:a + (1 << 20)], cs) - rs[None, :]
-                inside[a:a + (1 << 20)] |= d2.amin(dim=1) < 0
+            r2 = (rs * rs)[None, :]
+            step_pts = 1 << 18
+            for a in range(0, pts.shape[0], step_pts):
+                # exact squared distances (not the matrix-product form): the same labels on every run
+                d2 = ((pts[a:a + step_pts, None, :] - cs[None, :, :]) ** 2).sum(-1)
+                inside[a:a + step_pts] |= (d2 < r2).any(dim=1)
         fluid[i0:i1, j0:j1, k0:k1] = inside.reshape(i1 - i0, j1 - j0, k1 - k0).to(torch.uint8).cpu().numpy()
     # Surface mesh for the near-wall normals: rings around the centreline
     # (parallel-transported frames, outward triangle order) and the bulge sphere.

@@ -5,7 +5,7 @@ differences of the forward solver in FP64 (h = 1e-4 relative), by the FP64
 GPU adjoint for the FP32 one, and by the checkpoint schedule not changing
 them. Objective: time-averaged squared wall shear stress over a wall segment
 plus the inlet-outlet pressure drop; parameters: inlet-profile coefficients
-and a per-node Brinkman porosity field.
+and a per-node Brinkman porosity field. Every test runs on D3Q19 and D3Q27.
 """
 import numpy as np
 import pytest
@@ -20,11 +20,11 @@ T, K = 60, 16  # 3 full checkpoint intervals and a partial one
 M = 3
 
 
-@pytest.fixture(scope="module")
-def setup(tmp_path_factory):
+@pytest.fixture(scope="module", params=[19, 27], ids=["D3Q19", "D3Q27"])
+def setup(tmp_path_factory, request):
     wd = str(tmp_path_factory.mktemp("adj"))
     cfg = workloads.straight_tube(wd, radius=5.0, length=20.0, tau=TAU, cte_vel=0.03, max_iter=T)
-    geom = lbm.Geometry.from_config(cfg)
+    geom = lbm.Geometry.from_config(cfg, q=request.param)
     ijk, kind, _ = geom.nodes()
     inl = np.where(kind == lbm.KIND_INLET)[0]
     x = ijk[inl, 0] - ijk[inl, 0].mean()

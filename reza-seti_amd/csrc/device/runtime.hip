@@ -748,6 +748,7 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
   if (!g || !p || !out) return fail(LBM_ERR_ARG, "null argument");
   *out = nullptr;
   const Tables& t = g->t;
+  if (t.N == 0) return fail(LBM_ERR_ARG, "the geometry has no nodes");
   if (p->precision < 0 || p->precision > 2) return fail(LBM_ERR_ARG, "unknown precision");
   if (!(p->tau > 0.5)) return fail(LBM_ERR_ARG, "tau must exceed 0.5");
   if (p->wall_mode < 0 || p->wall_mode > 1 || p->inlet_mode < 0 || p->inlet_mode > 2 || p->outlet_mode < 0 ||

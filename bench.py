@@ -336,9 +336,18 @@ def cpu_baseline(geom, budget_s=12.0):
     t0 = time.perf_counter()
     ref.step(np.full(n, U_MEAN), nthreads=ncores)
     dt = time.perf_counter() - t0
+    # One-thread row: the reference's own loop is single-threaded per rank.
+    t0 = time.perf_counter()
+    ref.step(np.full(1, U_MEAN), nthreads=1)
+    per1 = time.perf_counter() - t0
+    n1 = int(max(1, min(20, 0.25 * budget_s / max(per1, 1e-6))))
+    t0 = time.perf_counter()
+    ref.step(np.full(n1, U_MEAN), nthreads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": round(geom.n * n / dt / 1e6, 3), "unit": "MLUPS", "cores": ncores, "kind": "port",
             "sample": f"{n} steps of the same {geom.n}-node bifurcation, oracle FP64 restatement of the "
-                      f"reference step (bounce-back walls, NEE inlet/outlet), OpenMP on {ncores} host threads"}
+                      f"reference step (bounce-back walls, NEE inlet/outlet), OpenMP on {ncores} host threads",
+            "single_thread": {"value": round(geom.n * n1 / dt1 / 1e6, 3), "steps": n1}}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "columns_step_bifurcation_f32_traffic.md")

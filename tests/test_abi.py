@@ -56,6 +56,11 @@ def test_invalid_solver_arguments_are_rejected_before_any_device_call():
         assert e.value.code == code, kw
     g27 = lbm.Geometry.from_labels(lab, origin, 1.0, periodic=(0, 0, 1), normals=False, q=27)
     assert g27.n == g.n
+    # copy probe: sizes below one element, no iterations, or a grid beyond 2^31 blocks
+    for nbytes, iters in ((8, 1), (1 << 20, 0), (1 << 47, 1)):
+        with pytest.raises(lbm.LbmError) as e:
+            lbm.copy_probe(nbytes, iters=iters)
+        assert e.value.code == lbm.ERR_ARG, (nbytes, iters)
 
 
 def test_geometry_edge_cases(tmp_path):

@@ -23,7 +23,10 @@
  *   Zou-He inlet        after the rule, rho and u_z of a -z inlet node equal
  *                       rhoN and U (Zou-He identities); compiled reference
  *                       Boundary::ApplyBc where oracle/_ref builds
- *   outlet / wall copy  hand-made 3-node cases
+ *   outlet / wall copy, hand-made node sets with answers derived by hand
+ *   stale links, 0.0    (tests/test_oracle_rules.py): a lone wall node, a
+ *   push rule           fluid-fluid-outlet and a fluid-wall-outlet column, a
+ *                       zero wall node beside a fluid node
  *   WSS                 plane shear closed form (tau tangential, analytic)
  *   voxeliser/classes   compiled reference Geometry/SubDomain where
  *                       oracle/_ref builds; analytic tube/sphere counts

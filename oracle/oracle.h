@@ -19,7 +19,8 @@
  *   collision           f = f_eq fixed point; moment identities sum f_eq = rho,
  *                       sum c f_eq = rho u; brute force vs a direct expansion
  *   streaming           mass/momentum conservation in a closed periodic box;
- *                       shift of a delta field by exactly c_q
+ *                       f_q(x + c_q) = f*_q(x) on a periodic box
+ *                       (tests/test_oracle_rules.py)
  *   Zou-He inlet        after the rule, rho and u_z of a -z inlet node equal
  *                       rhoN and U (Zou-He identities); compiled reference
  *                       Boundary::ApplyBc where oracle/_ref builds

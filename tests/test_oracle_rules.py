@@ -130,3 +130,23 @@ def test_a_zero_post_collision_value_is_not_streamed():
             assert np.all(fs_a == 0.0) and got == f0[1][0]
         else:
             assert fs_a[0] != 0.0 and got == fs_a[0]
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_streaming_shifts_each_population_by_its_velocity(q):
+    # Fully periodic 4 x 5 x 6 box of fluid nodes (reference order: i outer,
+    # k inner): every link exists and push streaming is a pure shift,
+    # f_q(x + c_q) = f*_q(x).
+    g = (4, 5, 6)
+    ijk = np.array([(i, j, k) for i in range(g[0]) for j in range(g[1]) for k in range(g[2])], np.int32)
+    n = len(ijk)
+    ref = oracle.Reference(q, g, ijk, np.zeros(n, np.uint8), np.full(n, -1, np.int32), np.zeros((0, 3)),
+                           (0.0, 0.0, -1.0), np.zeros(0, np.int32), np.zeros((0, 3)), TAU, 1.0, periodic=(1, 1, 1))
+    ref.set_f(_random_f(n, q, 7))
+    ref.step([0.0])
+    c, _, _ = oracle.lattice(q)
+    fs, f = ref.fs(), ref.f()
+    for qq in range(q):
+        t = (ijk + c[qq]) % np.array(g)
+        dst = (t[:, 0] * g[1] + t[:, 1]) * g[2] + t[:, 2]
+        assert np.array_equal(f[dst, qq], fs[:, qq]), qq

@@ -34,7 +34,9 @@
  *   normals             radial for a cylinder (tolerance); compiled reference
  *   extension modes     (half-way bounce-back, NEE inlet/outlet, Guo force)
  *                       have no reference counterpart: pinned by Poiseuille
- *                       closed form, boundary identities and conservation.
+ *                       closed form, boundary identities (NEE moments and
+ *                       non-equilibrium parts, tests/test_oracle_rules.py)
+ *                       and conservation.
  */
 #ifndef LBM_ORACLE_H
 #define LBM_ORACLE_H

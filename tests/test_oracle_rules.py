@@ -150,3 +150,42 @@ def test_streaming_shifts_each_population_by_its_velocity(q):
         t = (ijk + c[qq]) % np.array(g)
         dst = (t[:, 0] * g[1] + t[:, 1]) * g[2] + t[:, 2]
         assert np.array_equal(f[dst, qq], fs[:, qq]), qq
+
+
+def _moments(f, c):
+    rho = f.sum()
+    return rho, (f[:, None] * c).sum(0) / rho
+
+
+def _feq(rho, u, c, w):
+    cu = c @ u
+    return w * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * (u @ u))
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_non_equilibrium_extrapolation_identities(q):
+    # A z column, periodic in x and y: inlet (normal -z), two fluid nodes,
+    # outlet (normal +z). The extrapolated boundary nodes take the
+    # non-equilibrium part of the node behind them (which has every link, so
+    # its streamed populations are what the rule reads):
+    #   inlet:  density of the node behind, velocity U along +z;
+    #   outlet: the set density, velocity of the node behind.
+    U, rho_out = 0.03, 1.002
+    ijk = np.array([[0, 0, k] for k in range(4)], np.int32)
+    ref = oracle.Reference(q, (1, 1, 4), ijk, np.array([2, 0, 0, 3], np.uint8), np.array([-1, -1, -1, 0], np.int32),
+                           np.array([[0.0, 0.0, 1.0]]), (0.0, 0.0, -1.0), np.zeros(0, np.int32), np.zeros((0, 3)),
+                           TAU, 1.0, wall_mode=1, inlet_mode=1, outlet_mode=1, outlet_rho=rho_out, periodic=(1, 1, 0))
+    c, _, w = oracle.lattice(q)
+    c = c.astype(np.float64)
+    ref.set_f(_random_f(4, q, 8))
+    for step in range(3):
+        ref.step([U])
+        f = ref.f()
+        r1, u1 = _moments(f[1], c)
+        r0, u0 = _moments(f[0], c)
+        assert abs(r0 - r1) < 1e-15 and np.allclose(u0, [0.0, 0.0, U], rtol=0, atol=1e-15), step
+        assert np.allclose(f[0] - _feq(r1, np.array([0.0, 0.0, U]), c, w), f[1] - _feq(r1, u1, c, w), rtol=0, atol=1e-15)
+        r2, u2 = _moments(f[2], c)
+        r3, u3 = _moments(f[3], c)
+        assert abs(r3 - rho_out) < 1e-15 and np.allclose(u3, u2, rtol=0, atol=1e-15), step
+        assert np.allclose(f[3] - _feq(rho_out, u2, c, w), f[2] - _feq(r2, u2, c, w), rtol=0, atol=1e-15)

@@ -150,3 +150,41 @@ def test_reduced_precision_poiseuille(prec, tol):
     err = np.linalg.norm(up - u64) / np.linalg.norm(u64)
     print(f"precision {prec}: velocity rel L2 vs FP64 = {err:.3e}")
     assert err < tol
+
+
+@pytest.mark.parametrize("prec,tol", [(lbm.PREC_F64, 1e-9), (lbm.PREC_F32, 2e-5)])
+def test_brinkman_damping_momentum_budget_and_darcy_limit(prec, tol):
+    # Fully periodic box, uniform porosity alpha, body force F along z. The
+    # state stays uniform, and the Guo scheme with Brinkman damping changes
+    # the momentum per step by F_tot = F - alpha m, with the physical
+    # momentum m = (j + F/2) / (1 + alpha/2). So j_(n+1) = j_n + F - alpha m_n,
+    # which relaxes to the Darcy limit m = F / alpha (u = F / (alpha rho)).
+    # (The collision's own momentum change, omega (m - j) = omega F_tot / 2,
+    # and the forcing term (1 - omega/2) F_tot add up to F_tot for any tau.)
+    # Tolerances relative to F: FP64 rounding of the host-side momentum sums
+    # (~1e-16 on populations of order w_q); FP32 arithmetic on increments of
+    # order F.
+    alpha, g = 2.0 ** -7, 1e-5  # alpha exactly representable in the FP32 porosity input
+    geom = lbm.Geometry.from_labels(np.ones((6, 6, 6), np.uint8), (0.0, 0.0, 0.0), 1.0, periodic=(1, 1, 1),
+                                    normals=False)
+    sol = lbm.Solver(geom, tau=0.8, dt=DT, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NONE,
+                     outlet_mode=lbm.OUTLET_NONE, precision=prec, parity_checks=0, force=(0.0, 0.0, g), wss_every=0)
+    sol.set_porosity(np.full(geom.n, alpha))
+
+    def jz():
+        rho, u = sol.macroscopic()
+        j = rho * u[:, 2]
+        assert np.ptp(j) <= tol * g  # uniform state
+        return float(j.mean())
+
+    j = jz()
+    for n in range(5):
+        sol.step(1)
+        m = (j + 0.5 * g) / (1.0 + 0.5 * alpha)
+        expect = j + g - alpha * m
+        j = jz()
+        print(f"step {n + 1}: j = {j:.6e}, budget {expect:.6e}", flush=True)
+        assert abs(j - expect) <= tol * g
+    sol.step(4000)  # (1 - alpha / (1 + alpha / 2))^4000 ~ 3e-14
+    m = (jz() + 0.5 * g) / (1.0 + 0.5 * alpha)
+    assert m == pytest.approx(g / alpha, rel=100 * tol)

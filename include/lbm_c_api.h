@@ -248,14 +248,18 @@ int lbm_get_macroscopic(lbm_solver* s, double* rho, double* u);
 /* Simulation::WriteOutput (Simulation.cpp:835-856) with `dir` standing for
  * the working directory: dir/Fluid_Variables/<iter>.vtu
  * (OutputFile::WriteVTK_Collective, OutputFile.cpp:60-231: nodes in z-y-x
- * order at their grid indices, Velocity / Density / VelocityMagnitude of the
- * last collision, hexahedra over all-node cubes), dir/<iter>_FlowHemodynamics.csv
+ * order at their grid indices, Velocity / Density / VelocityMagnitude that
+ * the last iteration's collision set, i.e. of the populations before the
+ * last lbm_step, as WriteOutput(iter) sees them after iteration iter;
+ * hexahedra over all-node cubes), dir/<iter>_FlowHemodynamics.csv
  * (WriteSurfaceAttribute, :234-316: X,Y,Z,WSS[,OSI,TAWSS],VelMag per
- * near-wall node) and its copy smoothed over a 0.25 radius
+ * near-wall node; VelMag is 0 on D3Q27, where the reference never sets it)
+ * and its copy smoothed over a 0.25 radius
  * (processFlowData / smoothData, :369-483) in
  * dir/FlowHemodynamics/<phase_name>_<iter>_FlowHemodynamics.csv. OSI and
  * TAWSS columns when `pulsatile`. The VTU is XML with appended raw binary
- * data (the reference's VTK writer picks its own encoding). */
+ * data (the reference's VTK writer picks its own encoding). LBM_ERR_ARG if
+ * no step ran since lbm_create or lbm_set_populations. */
 int lbm_write_output(const lbm_geometry* g, lbm_solver* s, int32_t iter, const char* phase_name, const char* dir,
                      int32_t pulsatile);
 /* The two host-side pieces of lbm_write_output on their own: the VTU of given

@@ -85,6 +85,9 @@ void oracle_set_f(oracle_state* st, const double* f);    /* node-major N*Q */
 void oracle_get_f(const oracle_state* st, double* f);    /* m_distributions */
 void oracle_get_fs(const oracle_state* st, double* fs);  /* m_newDistributions */
 void oracle_get_feq(const oracle_state* st, double* feq);
+/* Density and velocity set by the last collision (node->density, node->velocity
+ * of Simulation::Collision, Simulation.cpp:77-139): N and 3N values. */
+void oracle_get_macro(const oracle_state* st, double* rho, double* u);
 /* wss, acc (3 per node), acc_mag in the setup's near-wall order. */
 void oracle_get_wss(const oracle_state* st, double* wss, double* acc, double* acc_mag);
 /* Runs nsteps iterations with inlet speed u_in[t]; nthreads <= 1 is serial.

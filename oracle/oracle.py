@@ -49,6 +49,7 @@ def lib():
             "oracle_get_f": ([_P, _P], None),
             "oracle_get_fs": ([_P, _P], None),
             "oracle_get_feq": ([_P, _P], None),
+            "oracle_get_macro": ([_P, _P, _P], None),
             "oracle_get_wss": ([_P, _P, _P, _P], None),
             "oracle_step": ([_P, C.c_int32, _P, C.c_int32], C.c_int),
             "oracle_collide": ([_P], None),
@@ -171,6 +172,12 @@ class Reference:
         out = np.zeros((self.n, self.q))
         lib().oracle_get_feq(self._h, _p(out))
         return out
+
+    def macro(self):
+        """(rho[N], u[N, 3]) set by the last collision."""
+        rho, u = np.zeros(self.n), np.zeros((self.n, 3))
+        lib().oracle_get_macro(self._h, _p(rho), _p(u))
+        return rho, u
 
     def set_f(self, f):
         f = np.ascontiguousarray(f, np.float64)

@@ -196,6 +196,11 @@ void oracle_get_f(const oracle_state* st, double* f) { memcpy(f, st->f, sizeof(d
 void oracle_get_fs(const oracle_state* st, double* fs) { memcpy(fs, st->fs, sizeof(double) * st->N * st->Q); }
 void oracle_get_feq(const oracle_state* st, double* feq) { memcpy(feq, st->feq, sizeof(double) * st->N * st->Q); }
 
+void oracle_get_macro(const oracle_state* st, double* rho, double* u) {
+  memcpy(rho, st->rho, sizeof(double) * st->N);
+  memcpy(u, st->u, sizeof(double) * 3 * st->N);
+}
+
 void oracle_get_wss(const oracle_state* st, double* wss, double* acc, double* accm) {
   if (wss) memcpy(wss, st->wss, sizeof(double) * st->n_nw);
   if (acc) memcpy(acc, st->acc, sizeof(double) * 3 * st->n_nw);

@@ -1091,13 +1091,15 @@ int lbm_get_populations(lbm_solver* s, double* f) {
   return LBM_OK;
 }
 
-int lbm_get_post_collision(lbm_solver* s, double* fs) {
-  if (!s || !fs) return fail(LBM_ERR_ARG, "null argument");
+namespace {
+// Post-collision state buffer `buf` (0 or 1) in reference node order, FP64
+// unshifted.
+int read_state_buffer(lbm_solver* s, int buf, double* fs) {
   HIP_TRY(hipSetDevice(s->prm.device));
   const int Q = s->Q;
   const size_t cnt = static_cast<size_t>(Q) * s->npad;
   std::vector<unsigned char> raw(cnt * s->esize);
-  HIP_TRY(hipMemcpyAsync(raw.data(), s->fs[s->cur], raw.size(), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipMemcpyAsync(raw.data(), s->fs[buf], raw.size(), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   const lbm::LatticeView L = lbm::lattice_view(Q);
   for (int64_t n = 0; n < s->N; ++n) {
@@ -1120,6 +1122,12 @@ int lbm_get_post_collision(lbm_solver* s, double* fs) {
     }
   }
   return LBM_OK;
+}
+}  // namespace
+
+int lbm_get_post_collision(lbm_solver* s, double* fs) {
+  if (!s || !fs) return fail(LBM_ERR_ARG, "null argument");
+  return read_state_buffer(s, s->cur, fs);
 }
 
 int lbm_get_wss(lbm_solver* s, double* wss, double* acc_tau, double* acc_mag) {
@@ -1645,14 +1653,18 @@ int lbm_write_output(const lbm_geometry* g, lbm_solver* s, int32_t iter, const c
   if (!g || !s || !phase_name || !dir) return fail(LBM_ERR_ARG, "null argument");
   const lbm::Tables& t = g->t;
   if (t.N != s->N || t.Q != s->Q) return fail(LBM_ERR_ARG, "geometry does not match the solver");
+  // WriteOutput(iter) follows iteration iter, whose Collision set each
+  // node's density and velocity from f(iter) (Simulation.cpp:77-139; wall
+  // nodes carry zero velocity in the reference wall rule). A step here
+  // streams f*(iter) into f(iter + 1) and then already collides that, so
+  // f*(iter) is the previous post-collision buffer, whose moments BGK keeps
+  // equal to those of f(iter) (to rounding). Before any step since the state
+  // was set, that buffer holds no collision.
+  if (!s->stepped) return fail(LBM_ERR_ARG, "no step since the state was set: output follows an iteration");
   const int Q = s->Q;
   const int64_t N = s->N;
-  // Density and velocity of the last collision: the moments of its output,
-  // which BGK leaves equal to those of its input (Simulation::Collision,
-  // Simulation.cpp:77-139; wall nodes carry zero velocity in the reference
-  // wall rule).
   std::vector<double> f(static_cast<size_t>(N) * Q);
-  int rc = lbm_get_post_collision(s, f.data());
+  int rc = read_state_buffer(s, 1 - s->cur, f.data());
   if (rc) return rc;
   const lbm::LatticeView L = lbm::lattice_view(Q);
   std::vector<double> rho(N), u(3 * N), vm(N);
@@ -1676,7 +1688,10 @@ int lbm_write_output(const lbm_geometry* g, lbm_solver* s, int32_t iter, const c
   std::vector<double> wss(m), osi(pulsatile ? m : 0), tawss(pulsatile ? m : 0), nvm(m);
   if ((rc = lbm_get_wss(s, wss.data(), nullptr, nullptr))) return rc;
   if (pulsatile && (rc = lbm_get_osi_tawss(s, osi.data(), tawss.data()))) return rc;
-  for (size_t i = 0; i < m; ++i) nvm[i] = vm[t.nearwall[i]];
+  // The CSV's VelMag is the node's velMag, which only the D3Q19 branch of
+  // Collision sets (Simulation.cpp:136-139); it stays at its initial 0 on
+  // D3Q27 (Node.h:64).
+  for (size_t i = 0; i < m; ++i) nvm[i] = Q == 19 ? vm[t.nearwall[i]] : 0.0;
   // Simulation::WriteOutput (Simulation.cpp:835-856), relative to `dir`.
   const std::string base(dir), it = std::to_string(iter);
   for (const char* sub : {"/Fluid_Variables", "/FlowHemodynamics"})

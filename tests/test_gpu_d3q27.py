@@ -102,3 +102,25 @@ def test_periodic_poiseuille_d3q27():
     coef, *_ = np.linalg.lstsq(A, uz[core], rcond=None)
     assert coef[1] == pytest.approx(-g / (4 * nu), rel=0.02)
     assert abs(sol.reduce().mass - s0.mass) < 1e-10 * s0.mass
+
+
+def test_d3q27_output_files(sten, tmp_path):
+    # The VTU carries the last iteration's collision as the oracle keeps it;
+    # the CSV's VelMag is 0, since only the D3Q19 collision sets it in the
+    # reference (Simulation.cpp:136-139).
+    from test_output import read_vtu
+
+    sol = lbm.Solver(sten, tau=TAU, dt=DT, cte_vel=U, inlet_mode=lbm.INLET_NONE)
+    ref = oracle.Reference.from_geometry(sten, tau=TAU, dt=DT, inlet_mode=2)
+    sol.step(25)
+    ref.step(np.full(25, U), nthreads=8)
+    sol.write_output(sten, 25, "Iteration_25", tmp_path, pulsatile=False)
+    rho, u = ref.macro()
+    ijk, _, _ = sten.nodes()
+    order = np.lexsort((ijk[:, 0], ijk[:, 1], ijk[:, 2]))
+    npts, _, a = read_vtu(tmp_path / "Fluid_Variables" / "25.vtu")
+    assert npts == sten.n
+    assert np.allclose(a["Density"], rho[order], rtol=1e-14, atol=0)
+    assert np.allclose(a["Velocity"], u[order], rtol=0, atol=1e-14)
+    raw = np.loadtxt(tmp_path / "25_FlowHemodynamics.csv", delimiter=",", skiprows=1, ndmin=2)
+    assert raw.shape == (sten.info.n_nearwall, 5) and np.all(raw[:, 4] == 0.0)

@@ -45,4 +45,20 @@ def test_driver_runs_the_reference_loop_bit_exact(tmp_path):
     assert raw[0] == "X,Y,Z,WSS,VelMag" and len(raw) == geom.info.n_nearwall + 1
     smooth = (out / "FlowHemodynamics" / "Iteration_100_100_FlowHemodynamics.csv").read_text().splitlines()
     assert smooth == raw  # pitch 1 > the 0.25 smoothing radius: no neighbours, values kept
-    assert (out / "Fluid_Variables" / "100.vtu").stat().st_size > 0
+    # Iteration_100 follows the 101st iteration: its fields are those that
+    # iteration's collision set, as the oracle keeps them after 101 steps.
+    from test_output import read_vtu
+
+    ref101 = oracle.Reference.from_geometry(geom, tau=0.8, dt=1e-5)
+    ref101.step(np.full(101, 0.03))
+    rho, u = ref101.macro()
+    ijk, _, _ = geom.nodes()
+    order = np.lexsort((ijk[:, 0], ijk[:, 1], ijk[:, 2]))
+    npts, _, a = read_vtu(out / "Fluid_Variables" / "100.vtu")
+    assert npts == geom.n
+    assert np.allclose(a["Density"], rho[order], rtol=1e-14, atol=0)
+    assert np.allclose(a["Velocity"], u[order], rtol=0, atol=1e-14)
+    vm = np.sqrt((u ** 2).sum(1))
+    nodes, _ = geom.nearwall()
+    col = np.loadtxt(out / "100_FlowHemodynamics.csv", delimiter=",", skiprows=1, ndmin=2)[:, 4]
+    assert np.allclose(col, vm[nodes], rtol=1e-5, atol=1e-15)

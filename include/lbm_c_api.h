@@ -63,6 +63,12 @@ extern "C" {
 #define LBM_FLOW_CONSTANT 0   /* CteVel every step (Simulation.cpp:558-559) */
 #define LBM_FLOW_PULSATILE 1  /* Simulation::getInletVelocity(t, 0.9) (Simulation.cpp:519-534, pulsatile.h) */
 
+/* Replaces Simulation::getInletVelocity (Simulation.cpp:519-534): the
+ * piecewise polynomial Pulsatile::InletVelequation1..3 (pulsatile.h:55-69)
+ * of fmod(t, period) on [0, 0.1p), [0.1p, 0.3p), [0.3p, 0.9p), else 0. This
+ * host function is the inlet speed lbm_step uses for LBM_FLOW_PULSATILE. */
+double lbm_inlet_velocity(double t, double period);
+
 /* Lattice tables: VelocitySet::GetDirection / GetWeight and
  * LatticeModel::OppositeDirection (VelocitySet.h:60-65, LatticeModel.h:50-54).
  * c: 3 ints per direction; q is 19 or 27. */
@@ -242,7 +248,11 @@ int lbm_get_osi_tawss(lbm_solver* s, double* osi, double* tawss);
 
 /* Density and velocity of f(t) per node (the values Simulation::Collision
  * computes at the start of the next iteration; wall velocity is 0 in
- * reference wall mode). */
+ * reference wall mode). These are the raw moments rho = sum f, u = sum c f / rho,
+ * as in the reference, which has no forcing. With the Guo force or a Brinkman
+ * field (extension modes) the physical velocity also takes half the step's
+ * force density, (sum c f + F/2) / rho; neither this call nor the VTU of
+ * lbm_write_output adds it. */
 int lbm_get_macroscopic(lbm_solver* s, double* rho, double* u);
 
 /* Simulation::WriteOutput (Simulation.cpp:835-856) with `dir` standing for
@@ -345,7 +355,10 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
  * and dJ/dalpha per node (either may be NULL). Inside each checkpoint
  * interval the states are recomputed on a binomial (revolve) schedule with
  * `slots` stored states (<= 0: as many as fit, at most checkpoint_every - 1).
- * The solver state afterwards is the forward pass's final state. */
+ * The solver state afterwards is the forward pass's final state. The record
+ * stays valid for further backward calls until lbm_step*, lbm_set_populations,
+ * lbm_set_inlet_profile or lbm_set_porosity runs; after any of them (or with
+ * no forward pass) this returns LBM_ERR_STATE. */
 int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double* grad_porosity);
 /* Moments tape of the reverse pass (default on): while recomputing a
  * checkpoint interval the step kernel stores the density and momentum sums

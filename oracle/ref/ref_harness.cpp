@@ -18,6 +18,7 @@
 #include "LBM.h"
 #include "LatticeModel.h"
 #include "Subdomain.h"
+#include "pulsatile.h"
 
 extern "C" {
 
@@ -135,6 +136,27 @@ int ref_lattice(int32_t* c, int32_t* opp, double* w) {
     w[q] = m.GetWeight(q);
   }
   return _nLatNodes;
+}
+
+// The reference's pulsatile waveform: Pulsatile::InletVelequation1..3 are
+// the reference header itself (pulsatile.h:49-71, compiled here unmodified).
+// Simulation::getInletVelocity (Simulation.cpp:519-534), which selects the
+// piece, lives in Simulation.cpp, which needs VTK and does not build here; its
+// eight lines of branch logic are restated below around the compiled pieces.
+double ref_inlet_velocity(double x, double period) {
+  Pulsatile p;
+  double vel = 0.0;
+  const double nx = fmod(x, period);
+  if (nx >= 0 && nx < (0.1 * period)) vel = p.InletVelequation1(nx);
+  else if (nx >= (0.1 * period) && nx < (0.3 * period)) vel = p.InletVelequation2(nx);
+  else if (nx >= (0.3 * period) && nx < (0.9 * period)) vel = p.InletVelequation3(nx);
+  return vel;
+}
+
+// One piece on its own (1, 2 or 3), for checks across the branch bounds.
+double ref_pulsatile_piece(int piece, double x) {
+  Pulsatile p;
+  return piece == 1 ? p.InletVelequation1(x) : piece == 2 ? p.InletVelequation2(x) : p.InletVelequation3(x);
 }
 
 int ref_zouhe(double vel, const double* normal3, double* f) {

@@ -61,6 +61,10 @@ def lib():
         L.ref_parse_config.restype = C.c_int
         L.ref_lattice.argtypes = [_P, _P, _P]
         L.ref_lattice.restype = C.c_int
+        L.ref_inlet_velocity.argtypes = [C.c_double, C.c_double]
+        L.ref_inlet_velocity.restype = C.c_double
+        L.ref_pulsatile_piece.argtypes = [C.c_int, C.c_double]
+        L.ref_pulsatile_piece.restype = C.c_double
         _lib = L
     return _lib
 
@@ -107,6 +111,16 @@ def parse_config(path):
     if lib().ref_parse_config(str(path).encode(), C.byref(c)) != 0:
         raise RuntimeError("reference parser failed")
     return c
+
+
+def inlet_velocity(x, period=0.9):
+    """Pulsatile::InletVelequation1..3 of the reference header, selected as
+    Simulation::getInletVelocity does."""
+    return lib().ref_inlet_velocity(float(x), float(period))
+
+
+def pulsatile_piece(piece, x):
+    return lib().ref_pulsatile_piece(int(piece), float(x))
 
 
 def lattice():

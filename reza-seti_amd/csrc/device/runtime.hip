@@ -28,29 +28,6 @@ using namespace lbm;
 
 namespace {
 
-// Pulsatile::InletVelequation1..3 (pulsatile.h:55-69).
-double pulse1(double x) {
-  return (-8.579e+08 * pow(x, 8) + 4.059e+08 * pow(x, 7) - 7.817e+07 * pow(x, 6) + 7.994e+06 * pow(x, 5) -
-          4.835e+05 * pow(x, 4) + 1.782e+04 * pow(x, 3) - 321.6 * pow(x, 2) + 3.215 * x + 0.1423);
-}
-double pulse2(double x) {
-  return (-4.32e+06 * pow(x, 8) + 7.94e+06 * pow(x, 7) - 6.298e+06 * pow(x, 6) + 2.82e+06 * pow(x, 5) -
-          7.809e+05 * pow(x, 4) + 1.37e+05 * pow(x, 3) - 1.484e+04 * pow(x, 2) + 901.9 * x - 22.87);
-}
-double pulse3(double x) {
-  return (-5.123 * pow(x, 4) + 14.15 * pow(x, 3) - 14.21 * pow(x, 2) + 5.865 * x - 0.5696);
-}
-
-// Simulation::getInletVelocity (Simulation.cpp:519-534).
-double inlet_velocity(double x, double period) {
-  double vel = 0.0;
-  const double nx = fmod(x, period);
-  if (nx >= 0 && nx < (0.1 * period)) vel = pulse1(nx);
-  else if (nx >= (0.1 * period) && nx < (0.3 * period)) vel = pulse2(nx);
-  else if (nx >= (0.3 * period) && nx < (0.9 * period)) vel = pulse3(nx);
-  return vel;
-}
-
 float half_to_float(uint16_t h) {
   const uint32_t s = (h >> 15) & 1u, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
   float v;
@@ -165,7 +142,7 @@ struct lbm_solver {
   // Adjoint forward record (lbm_adjoint_forward).
   struct Adjoint {
     bool recorded = false;
-    int32_t T = 0, K = 0, nck = 0, n_in = 0, n_out = 0;
+    int32_t T = 0, K = 0, M = 0, nck = 0, n_in = 0, n_out = 0;  // M: inlet modes of the record
     std::vector<void*> ckpt;          // h_0, h_K, h_2K, ..., h_T
     void* prev = nullptr;             // h_{T-1} (the input of the last step)
     std::vector<double> coef;         // T * n_modes
@@ -347,6 +324,9 @@ int column_step(lbm_solver* s, ColArgs& c, bool force) {
 int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
   if (nsteps < 0) return fail(LBM_ERR_ARG, "nsteps < 0");
   HIP_TRY(hipSetDevice(s->prm.device));
+  // A step leaves the recorded forward pass: its reverse pass would restore
+  // the recorded final state under the later iteration count and sums.
+  if (nsteps > 0) adjoint_release(s);
   const bool force = has_force(s);
   s->flush_used = 0;
   s->flush_count = 0;
@@ -361,7 +341,7 @@ int run_steps(lbm_solver* s, int32_t nsteps, const double* coef) {
     if (coef) {
       for (int m = 0; m < s->n_modes; ++m) a.coef[m] = coef[int64_t(t) * s->n_modes + m];
     } else {
-      const double U = (s->prm.flow_type == LBM_FLOW_PULSATILE) ? inlet_velocity(s->total_runtime, 0.9)
+      const double U = (s->prm.flow_type == LBM_FLOW_PULSATILE) ? lbm_inlet_velocity(s->total_runtime, 0.9)
                                                                  : s->prm.cte_vel;
       for (int m = 0; m < MAX_MODES; ++m) a.coef[m] = 0.0;
       a.coef[0] = U;
@@ -947,6 +927,7 @@ int lbm_set_inlet_profile(lbm_solver* s, int32_t n_modes, const double* basis) {
   const size_t ninl = s->inlet_nodes.size();
   if (ninl == 0) return fail(LBM_ERR_ARG, "no inlet nodes");
   HIP_TRY(hipSetDevice(s->prm.device));
+  adjoint_release(s);  // the recorded forward pass used the previous profile
   HIP_TRY(hipMemcpyAsync(s->inlet_basis, basis, n_modes * ninl * sizeof(double), hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->n_modes = n_modes;
@@ -957,6 +938,7 @@ int lbm_set_inlet_profile(lbm_solver* s, int32_t n_modes, const double* basis) {
 int lbm_set_porosity(lbm_solver* s, const float* alpha) {
   if (!s) return fail(LBM_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->prm.device));
+  adjoint_release(s);  // the recorded forward pass used the previous field
   // Before the first step the initial collision is redone with the new
   // field, as if it had been given at creation.
   auto recollide = [&]() -> int {
@@ -1046,6 +1028,7 @@ int lbm_last_batch_kernel_time(lbm_solver* s, double* step_ms, double* other_ms)
 int lbm_set_populations(lbm_solver* s, const double* f) {
   if (!s || !f) return fail(LBM_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->prm.device));
+  adjoint_release(s);  // a new state ends the recorded forward pass
   int rc = init_state(s, f);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1283,6 +1266,7 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
   s->last_set.clear();
   A.T = T;
   A.K = K;
+  A.M = M;
   A.nck = (T + K - 1) / K + 1;
   A.ckpt.assign(A.nck, nullptr);
   // Checkpoints h_0, h_K, ..., h_T and the last step's input h_{T-1}, in a
@@ -1312,7 +1296,7 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
     if (coef) {
       for (int m = 0; m < M; ++m) c.coef[m] = coef[int64_t(t) * M + m];
     } else {
-      c.coef[0] = (s->prm.flow_type == LBM_FLOW_PULSATILE) ? inlet_velocity(s->total_runtime, 0.9) : s->prm.cte_vel;
+      c.coef[0] = (s->prm.flow_type == LBM_FLOW_PULSATILE) ? lbm_inlet_velocity(s->total_runtime, 0.9) : s->prm.cte_vel;
     }
     for (int m = 0; m < M; ++m) A.coef[int64_t(t) * M + m] = c.coef[m];
     c.wss_on = 1;
@@ -1355,7 +1339,7 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
   if (!A.recorded) return fail(LBM_ERR_STATE, "no adjoint forward pass recorded");
   HIP_TRY(hipSetDevice(s->prm.device));
   const int Q = s->Q;
-  const int32_t T = A.T, K = A.K, M = s->n_modes;
+  const int32_t T = A.T, K = A.K, M = A.M;
   const int64_t npad = s->npad;
   const size_t state_bytes = static_cast<size_t>(Q) * npad * s->esize;
   const size_t aes = s->prec == 0 ? 8 : 4;  // adjoint arithmetic: FP64 or FP32

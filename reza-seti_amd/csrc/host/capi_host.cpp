@@ -1,4 +1,5 @@
 // Host half of the C ABI: config parsing and geometry/node tables.
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <stdexcept>
@@ -65,6 +66,24 @@ using namespace lbm;
 extern "C" {
 
 const char* lbm_last_error(void) { return g_last_error.c_str(); }
+
+// Simulation::getInletVelocity (Simulation.cpp:519-534) over the three
+// polynomial pieces of Pulsatile::InletVelequation1..3 (pulsatile.h:55-69),
+// with the reference's terms, powers and branch bounds.
+double lbm_inlet_velocity(double t, double period) {
+  const double x = std::fmod(t, period);
+  if (x >= 0 && x < (0.1 * period))
+    return -8.579e+08 * std::pow(x, 8) + 4.059e+08 * std::pow(x, 7) - 7.817e+07 * std::pow(x, 6) +
+           7.994e+06 * std::pow(x, 5) - 4.835e+05 * std::pow(x, 4) + 1.782e+04 * std::pow(x, 3) -
+           321.6 * std::pow(x, 2) + 3.215 * x + 0.1423;
+  if (x >= (0.1 * period) && x < (0.3 * period))
+    return -4.32e+06 * std::pow(x, 8) + 7.94e+06 * std::pow(x, 7) - 6.298e+06 * std::pow(x, 6) +
+           2.82e+06 * std::pow(x, 5) - 7.809e+05 * std::pow(x, 4) + 1.37e+05 * std::pow(x, 3) -
+           1.484e+04 * std::pow(x, 2) + 901.9 * x - 22.87;
+  if (x >= (0.3 * period) && x < (0.9 * period))
+    return -5.123 * std::pow(x, 4) + 14.15 * std::pow(x, 3) - 14.21 * std::pow(x, 2) + 5.865 * x - 0.5696;
+  return 0.0;
+}
 
 int lbm_lattice(int32_t q, int32_t* c, int32_t* opp, double* w) {
   if (q != 19 && q != 27) return fail(LBM_ERR_ARG, "q must be 19 or 27");

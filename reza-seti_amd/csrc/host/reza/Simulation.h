@@ -119,7 +119,10 @@ class Simulation {
   }
 
   // One reference iteration per call (see the file comment).
-  void Collision() { check(lbm_step(solver_.get(), 1), "Collision"); }
+  void Collision() {
+    check(lbm_step(solver_.get(), 1), "Collision");
+    osi_current_ = false;
+  }
   void Communicate() {}      // one GPU: nothing to exchange (Simulation.cpp:270-516)
   void Streaming() {}        // folded into Collision()
   void ApplyZouHeBc() {}     // folded into Collision()
@@ -127,30 +130,24 @@ class Simulation {
   void ApplyWallBc() {}      // folded into Collision()
   void ComputeWallShearStress(int) {}  // accumulated by the fused launch
 
-  // Simulation::ComputeOSIAndTAWSS (Simulation.cpp:804-826).
-  void ComputeOSIAndTAWSS(int) {
+  // Simulation::ComputeOSIAndTAWSS (Simulation.cpp:804-826). OSI and TAWSS
+  // are a pure function of the stress accumulators, so the per-iteration call
+  // only marks them out of date; FetchOSIAndTAWSS (called by WriteOutput)
+  // reads them from the device when they are used, instead of a device
+  // round trip every iteration.
+  void ComputeOSIAndTAWSS(int) { osi_current_ = false; }
+  void FetchOSIAndTAWSS() {
+    if (osi_current_) return;
     const int64_t n = SubDomain_.Geometry_.info().n_nearwall;
     osi.resize(n);
     tawss.resize(n);
     check(lbm_get_osi_tawss(solver_.get(), osi.data(), tawss.data()), "ComputeOSIAndTAWSS");
+    osi_current_ = true;
   }
 
-  // Simulation::getInletVelocity (Simulation.cpp:519-534, pulsatile.h:55-69).
-  double getInletVelocity(double x, double period) {
-    double vel = 0.0;
-    const double t = std::fmod(x, period);
-    if (t >= 0 && t < (0.1 * period))
-      vel = (-8.579e+08 * std::pow(t, 8) + 4.059e+08 * std::pow(t, 7) - 7.817e+07 * std::pow(t, 6) +
-             7.994e+06 * std::pow(t, 5) - 4.835e+05 * std::pow(t, 4) + 1.782e+04 * std::pow(t, 3) -
-             321.6 * std::pow(t, 2) + 3.215 * t + 0.1423);
-    else if (t >= (0.1 * period) && t < (0.3 * period))
-      vel = (-4.32e+06 * std::pow(t, 8) + 7.94e+06 * std::pow(t, 7) - 6.298e+06 * std::pow(t, 6) +
-             2.82e+06 * std::pow(t, 5) - 7.809e+05 * std::pow(t, 4) + 1.37e+05 * std::pow(t, 3) -
-             1.484e+04 * std::pow(t, 2) + 901.9 * t - 22.87);
-    else if (t >= (0.3 * period) && t < (0.9 * period))
-      vel = (-5.123 * std::pow(t, 4) + 14.15 * std::pow(t, 3) - 14.21 * std::pow(t, 2) + 5.865 * t - 0.5696);
-    return vel;
-  }
+  // Simulation::getInletVelocity (Simulation.cpp:519-534, pulsatile.h:55-69):
+  // the same host function lbm_step evaluates for a pulsatile inlet.
+  double getInletVelocity(double x, double period) { return lbm_inlet_velocity(x, period); }
 
   void SetRelaxation(double tau) { m_relaxation = tau; }
 
@@ -196,7 +193,7 @@ class Simulation {
   void WriteOutput(int iter, const std::string& phase_name) {
     if (output_dir.empty()) return;
     const bool pulsatile = simParams.FlowType == "Pulsatile";
-    if (pulsatile) ComputeOSIAndTAWSS(iter);
+    if (pulsatile) FetchOSIAndTAWSS();
     check(lbm_write_output(SubDomain_.Geometry_.handle(), solver_.get(), iter, phase_name.c_str(),
                            output_dir.c_str(), pulsatile ? 1 : 0),
           "WriteOutput");
@@ -215,6 +212,7 @@ class Simulation {
 
  private:
   std::shared_ptr<lbm_solver> solver_;
+  bool osi_current_ = false;
 };
 
 }  // namespace reza

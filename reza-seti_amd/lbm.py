@@ -91,6 +91,7 @@ _lib = None
 _P = C.c_void_p
 SIGNATURES = [
     ("lbm_lattice", [C.c_int32, _P, _P, _P], C.c_int),
+    ("lbm_inlet_velocity", [C.c_double, C.c_double], C.c_double),
     ("lbm_config_parse", [C.c_char_p, _P], C.c_int),
     ("lbm_geometry_options_default", [_P], None),
     ("lbm_geometry_from_config", [C.c_char_p, _P, _P], C.c_int),
@@ -185,6 +186,11 @@ def lattice(q=19):
     w = np.zeros(q, np.float64)
     _check(lib().lbm_lattice(q, _ptr(c), _ptr(opp), _ptr(w)))
     return c, opp, w
+
+
+def inlet_velocity(t, period=0.9):
+    """Simulation::getInletVelocity: the pulsatile inlet speed lbm_step uses."""
+    return lib().lbm_inlet_velocity(float(t), float(period))
 
 
 def parse_config(path):

@@ -112,6 +112,39 @@ def straight_tube(workdir, radius=10.0, length=80.0, tau=0.8, cte_vel=0.05, max_
     return cfg
 
 
+_AXES = {"x": (2, 0, 1), "y": (1, 2, 0), "z": (0, 1, 2)}  # new coordinate a takes old coordinate _AXES[ax][a]
+
+
+def axis_tube(workdir, axis="z", inlet_end="min", radius=5.0, length=24.0, tau=0.8, cte_vel=0.02, max_iter=200,
+              dt=1e-5, flow_type="Constant", segments=64):
+    """Straight tube along `axis` (the z tube of tube_surface under a cyclic
+    coordinate permutation, which keeps the triangles' outward orientation).
+    The inlet is the cap at the `inlet_end` ("min" or "max") of the axis, with
+    its outward normal, and the outlet the opposite cap, so the six
+    (axis, end) pairs select each branch of the reference's Zou-He rule
+    (Boundary.cpp:111-289) and of its outlet copy (Simulation.cpp:650-741)
+    once. The end caps lie parallel to the voxeliser's +z rays for the x and y
+    tubes; such triangles never count (ComplexGeometry.cpp:291). Returns the
+    config path."""
+    os.makedirs(workdir, exist_ok=True)
+    perm = _AXES[axis]
+    tris, nrm = tube_surface(radius, length, segments=segments)
+    tris, nrm = tris[:, :, perm], nrm[:, perm]
+    a = "xyz".index(axis)
+    lo = [0.0, 0.0, 0.0]
+    lo[a] = -1.0
+    hi = [0.0, 0.0, 0.0]
+    hi[a] = 1.0
+    ends = {"min": dict(plane=axis, coordinate=0.0, normal=tuple(int(v) for v in lo)),
+            "max": dict(plane=axis, coordinate=float(length), normal=tuple(int(v) for v in hi))}
+    stl = os.path.join(workdir, f"tube_{axis}.stl")
+    write_stl(stl, tris, nrm, f"tube_{axis}")
+    cfg = os.path.join(workdir, "config.txt")
+    write_config(cfg, os.path.basename(stl), tau=tau, cte_vel=cte_vel, max_iter=max_iter, dt=dt, flow_type=flow_type,
+                 inlets=[ends[inlet_end]], outlets=[ends["max" if inlet_end == "min" else "min"]])
+    return cfg
+
+
 def revolution_surface(radius_of_z, z_levels, segments=128, center=(0.37, 0.21)):
     """Closed surface of revolution about the z axis through `center`:
     radius_of_z(z) sampled at the increasing z_levels, rims at half-segment

@@ -62,3 +62,61 @@ def test_driver_runs_the_reference_loop_bit_exact(tmp_path):
     nodes, _ = geom.nearwall()
     col = np.loadtxt(out / "100_FlowHemodynamics.csv", delimiter=",", skiprows=1, ndmin=2)[:, 4]
     assert np.allclose(col, vm[nodes], rtol=1e-5, atol=1e-15)
+
+
+def _pulsatile_schedule(dt, systole, diastole):
+    """Simulation::Iterate's Pulsatile branch (Simulation.cpp:549-602) restated
+    over the loop variables alone: the iterations, the inlet speed of each
+    and the (phase, iteration) of every output."""
+    t, it, speeds, outs, ns, nd = 0.0, 0, [], [], 0, 0
+    while t <= 2.7:
+        speeds.append(oracle.inlet_velocity(t, 0.9))
+        tic = np.fmod(t, 0.9)
+        if abs(tic - systole) < dt / 2:
+            ns += 1
+            outs.append((f"Systole_{ns}", it))
+        elif abs(tic - diastole) < dt / 2:
+            nd += 1
+            outs.append((f"Diastole_{nd}", it))
+        t += dt
+        it += 1
+    return it, np.array(speeds), outs
+
+
+@pytest.mark.gpu
+def test_driver_runs_the_pulsatile_loop(tmp_path):
+    import lbm
+
+    dt = 0.01
+    cfg = workloads.axis_tube(str(tmp_path), axis="z", radius=5.0, length=16.0, tau=1.0, dt=dt,
+                              flow_type="Pulsatile")
+    n, speeds, outs = _pulsatile_schedule(dt, 0.15, 0.9)
+    assert any(p.startswith("Systole") for p, _ in outs) and any(p.startswith("Diastole") for p, _ in outs)
+    dump, out = tmp_path / "f.bin", tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([EXE, cfg, "--dump", str(dump), "--out", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert f"Iterations: {n}," in r.stdout  # the t <= 2.7 stop
+    got = sorted(os.listdir(out / "FlowHemodynamics"))
+    assert got == sorted(f"{p}_{it}_FlowHemodynamics.csv" for p, it in outs)
+    assert sorted(os.listdir(out / "Fluid_Variables")) == sorted(f"{it}.vtu" for _, it in outs)
+    geom = lbm.Geometry.from_config(cfg)
+    ref = oracle.Reference.from_geometry(geom, tau=1.0, dt=dt)
+    ref.step(speeds, nthreads=8)
+    f = np.fromfile(dump, dtype=np.float64).reshape(geom.n, 19)
+    assert np.array_equal(f, ref.f())
+    # the last output's OSI and TAWSS columns from the oracle's accumulators
+    # at that iteration (Simulation.cpp:813-824)
+    phase, it = outs[-1]
+    ref2 = oracle.Reference.from_geometry(geom, tau=1.0, dt=dt)
+    ref2.step(speeds[: it + 1], nthreads=8)
+    _, acc, mag = ref2.wss()
+    osi = 0.5 * (1.0 - np.sqrt((acc ** 2).sum(1)) / mag)
+    tawss = mag / 0.9 * dt
+    raw = np.loadtxt(out / f"{it}_FlowHemodynamics.csv", delimiter=",", skiprows=1, ndmin=2)
+    head = (out / f"{it}_FlowHemodynamics.csv").read_text().splitlines()[0]
+    assert head == "X,Y,Z,WSS,OSI,TAWSS,VelMag"
+    np.testing.assert_allclose(raw[:, 4], osi, rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(raw[:, 5], tawss, rtol=1e-5, atol=0)

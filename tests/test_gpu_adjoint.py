@@ -195,3 +195,31 @@ def test_torch_autograd_function(setup):
     f = lambda th: Objective.apply(th, alpha.detach())  # noqa: E731
     assert torch.autograd.gradcheck(f, (theta.detach().clone().requires_grad_(True),), eps=1e-7, atol=1e-8,
                                     rtol=1e-3)
+
+
+def test_backward_refuses_a_changed_solver(setup):
+    # The reverse pass belongs to the recorded forward pass: a later step,
+    # new populations, a new inlet profile or a new porosity field ends the
+    # record, and the backward call then reports LBM_ERR_STATE instead of
+    # returning gradients of another problem.
+    st = setup
+    coef = st["s"][:, None] * st["theta"][None, :]
+    geom = st["geom"]
+    _, _, w = lbm.lattice(geom.q)
+    changes = {
+        "step": lambda s: s.step(1),
+        "populations": lambda s: s.set_populations(np.tile(w, (geom.n, 1))),
+        "inlet profile": lambda s: s.set_inlet_profile(np.vstack([st["basis"], st["basis"][:1]])),
+        "porosity": lambda s: s.set_porosity(st["alpha"]),
+    }
+    for name, change in changes.items():
+        sol = make(st)
+        sol.adjoint_forward(T, coef=coef, wss_weight=st["wss_w"], dp_weight=st["dp_w"], wss_mask=st["mask"],
+                            checkpoint_every=K)
+        g1, _ = sol.adjoint_backward()
+        g2, _ = sol.adjoint_backward()  # the record survives a backward pass
+        assert np.array_equal(g1, g2)
+        change(sol)
+        with pytest.raises(lbm.LbmError) as e:
+            sol.adjoint_backward()
+        assert e.value.code == lbm.ERR_STATE, name

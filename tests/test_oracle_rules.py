@@ -189,3 +189,25 @@ def test_non_equilibrium_extrapolation_identities(q):
         r3, u3 = _moments(f[3], c)
         assert abs(r3 - rho_out) < 1e-15 and np.allclose(u3, u2, rtol=0, atol=1e-15), step
         assert np.allclose(f[3] - _feq(rho_out, u2, c, w), f[2] - _feq(r2, u2, c, w), rtol=0, atol=1e-15)
+
+
+def test_a_zero_rest_population_keeps_its_value():
+    # A lone plain fluid node with f_rest = 3 and every other population 0,
+    # at tau = 2/3 (1/tau = 1.5 exactly): rho = 3, u = 0, f_eq,rest =
+    # w_rest * 3 = 1 and f*_rest = 3 + 1.5 (1 - 3) = 0.0 exactly, so the
+    # rest population, whose link points at the node itself, is not pushed
+    # and keeps 3 (Simulation.cpp:245-251); with no neighbours every other
+    # population keeps its 0 as well. A push of zeros would leave f_rest = 0.
+    q = 19
+    r = _rest(q)
+    ref = oracle.Reference(q, (1, 1, 1), np.zeros((1, 3), np.int32), np.zeros(1, np.uint8),
+                           np.full(1, -1, np.int32), np.zeros((0, 3)), (0.0, 0.0, -1.0), np.zeros(0, np.int32),
+                           np.zeros((0, 3)), 2.0 / 3.0, 1.0, wall_mode=0, inlet_mode=0, outlet_mode=0)
+    f = np.zeros((1, q))
+    f[0, r] = 3.0
+    ref.set_f(f)
+    ref.collide()
+    fs = ref.fs()[0]
+    assert fs[r] == 0.0 and np.all(fs[np.arange(q) != r] != 0.0)
+    ref.step([0.0])
+    assert np.array_equal(ref.f(), f)

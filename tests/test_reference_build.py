@@ -154,3 +154,53 @@ def test_zou_he_equals_reference_bit_for_bit(normal):
         U = 0.08 * rng.uniform(-1, 1)
         # the -y branch divides by (1 - 1.0) (Boundary.cpp:206): inf/nan on both sides
         assert np.array_equal(oracle.zouhe(f, U, normal), refbind.zouhe(f, U, normal), equal_nan=True)
+
+
+def _waveform_times():
+    p = 0.9
+    ts = list(np.linspace(0.0, 2.7, 40001))
+    for k in range(4):
+        for b in (0.0, 0.1 * p, 0.3 * p, 0.9 * p):
+            x = k * p + b
+            ts += [x, np.nextafter(x, -np.inf), np.nextafter(x, np.inf)]
+    for dt in (1e-5, 2e-3, 0.01, 0.05):  # the reference's accumulated total_runtime (Simulation.cpp:598-600)
+        t = 0.0
+        while t <= 2.7:
+            ts.append(t)
+            t += dt
+    return np.array(ts)
+
+
+def test_inlet_waveform_equals_reference_bit_for_bit():
+    # Pulsatile::InletVelequation1..3 compiled from the reference header, the
+    # product's lbm_inlet_velocity (what lbm_step and reza::Simulation use)
+    # and the oracle's restatement, on a dense sweep of [0, 2.7], every
+    # branch bound and its neighbouring doubles, and the times the
+    # reference's t += dt accumulation produces.
+    ts = _waveform_times()
+    seen = set()
+    for t in ts:
+        r = refbind.inlet_velocity(t, 0.9)
+        assert lbm.inlet_velocity(t, 0.9) == r, t
+        assert oracle.inlet_velocity(t, 0.9) == r, t
+        x = np.fmod(t, 0.9)
+        seen.add(0 if x < 0.09 else 1 if x < 0.27 else 2 if x < 0.81 else 3)
+    assert seen == {0, 1, 2, 3}
+    for period in (0.8, 1.0):  # the bounds scale with the period argument
+        for t in np.linspace(0.0, 3.0, 3001):
+            assert lbm.inlet_velocity(t, period) == refbind.inlet_velocity(t, period)
+
+
+def test_inlet_waveform_pieces_meet_the_branch_bounds():
+    # The selected piece at each bound is the one the reference's half-open
+    # intervals pick: [0, 0.09) eq1, [0.09, 0.27) eq2, [0.27, 0.81) eq3,
+    # [0.81, 0.9) zero.
+    p = 0.9
+    assert lbm.inlet_velocity(0.0, p) == refbind.pulsatile_piece(1, 0.0) == 0.1423
+    for b, piece in ((0.1 * p, 2), (0.3 * p, 3)):
+        assert lbm.inlet_velocity(b, p) == refbind.pulsatile_piece(piece, b)
+        below = np.nextafter(b, 0.0)
+        assert lbm.inlet_velocity(below, p) == refbind.pulsatile_piece(piece - 1, below)
+    assert lbm.inlet_velocity(0.9 * p, p) == 0.0
+    below = np.nextafter(0.9 * p, 0.0)
+    assert lbm.inlet_velocity(below, p) == refbind.pulsatile_piece(3, below)

@@ -12,6 +12,11 @@ A "step" is one fused kernel launch = one reference iteration over all nodes.
 Inputs are resident in HBM when the timed region starts.
 
     python bench.py [--gpus 1] [--steps 8000] [--warmup 100]
+
+The FP32/FP16-vs-FP64 errors are taken after the workload's own 8000 steps
+(--precision-steps), whatever --steps is; that FP32 run is also the
+sustained measurement behind `roofline` (its mean step-kernel launch time),
+and the --steps batch is reported beside it as `roofline.burst`.
 """
 import argparse
 import json
@@ -79,12 +84,22 @@ def tau_for():
     return 3.0 * nu + 0.5
 
 
-def run_precision(geom, basis, prec, steps, warmup):
+PRECISION_STEPS = 8000  # the precision study's own horizon: two Womersley periods
+
+
+def bifurcation_solver(geom, basis, prec):
     s = lbm.Solver(geom, precision=prec, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NEE_VELOCITY,
                    outlet_mode=lbm.OUTLET_NEE_PRESSURE, tau=tau_for(), dt=1e-5, parity_checks=0,
                    wss_every=WSS_EVERY)
     s.set_inlet_profile(basis)
     s.set_kernel_timing(True)
+    return s
+
+
+def run_precision(geom, basis, prec, steps, warmup):
+    """warmup untimed steps, then `steps` timed in one batch (HIP events on
+    the solver's stream; the stress-flush launches timed apart)."""
+    s = bifurcation_solver(geom, basis, prec)
     if warmup:
         s.step_coefficients(coefficients(0, warmup))
     s.synchronize()
@@ -97,6 +112,33 @@ def run_precision(geom, basis, prec, steps, warmup):
     step_ms, flush_ms = s.last_batch_kernel_ms()
     assert n == steps
     return s, wall, ev_ms, step_ms, flush_ms
+
+
+REF_FAMILY_STEPS = 500
+
+
+def run_reference_family(geom, steps=REF_FAMILY_STEPS, warmup=20):
+    """The reference-parity kernel family on the same vessel: FP64, the
+    reference's own rules (Zou-He inlet at the constant speed U_mean, outlet
+    copy, sequential wall copy, f* == 0 push check, WSS every step), i.e.
+    lbm_params_default as a drop-in for Simulation::Iterate runs it."""
+    s = lbm.Solver(geom, tau=tau_for(), dt=1e-5, cte_vel=U_MEAN)
+    s.step(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms, _ = s.last_batch_ms()
+    info = s.info()
+    launch_s = ev_ms / 1e3 / steps
+    achieved = geom.n * info.bytes_per_update / launch_s / 1e9
+    s.close()
+    return {"workload": "same vessel, FP64, reference rules (Zou-He inlet, outlet copy, wall copy rule, "
+                        "f* == 0 push check, WSS every step): the lbm_kernel family",
+            "steps": steps, "mlups": round(geom.n * steps / wall / 1e6, 1), "launch_us": round(launch_s * 1e6, 2),
+            "special_nodes": int(info.n_special), "achieved_gbs": round(achieved, 1),
+            "roofline_frac": round(achieved / PEAK_HBM_GBS, 4)}
 
 
 ADJ_STEPS, ADJ_EVERY, ADJ_MODES = 4000, 200, 16
@@ -320,45 +362,78 @@ def large_share_child(steps, timeout_s=900):
     return res
 
 
+def host_description():
+    """Core counts and CPU model of the host this runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"cpu_model": model, "host_cpus": os.cpu_count(), "cpus_available": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads():
+    """The host's CPU share for this process: the cores it may run on, at most
+    OMP_NUM_THREADS where that is set (the share of one GPU on a shared
+    node)."""
+    host = host_description()
+    n = host["cpus_available"] or 1
+    if host["omp_num_threads"]:
+        n = min(n, int(host["omp_num_threads"]))
+    return max(1, n), host
+
+
 def cpu_baseline(geom, budget_s=12.0):
-    """The oracle (FP64 restatement of the reference step, OpenMP) on the host
-    cores, on a bounded sample of the same workload."""
+    """The oracle (FP64 restatement of the reference step with the
+    reference's own boundary rules: Zou-He inlet, outlet copy, wall copy,
+    wall shear stress every step, Simulation.cpp:553-568) on the host's
+    cores, on a bounded sample of the same workload, plus a one-thread row
+    (one reference MPI rank runs single-threaded)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    ncores = min(16, os.cpu_count() or 1)
-    ijk, kind, _ = geom.nodes()
-    ref = oracle.Reference.from_geometry(geom, tau=tau_for(), dt=1e-5, wall_mode=1, inlet_mode=1, outlet_mode=1)
+    ncores, host = cpu_threads()
+    ref = oracle.Reference.from_geometry(geom, tau=tau_for(), dt=1e-5, wall_mode=0, inlet_mode=0, outlet_mode=0)
     t0 = time.perf_counter()
     ref.step(np.full(2, U_MEAN), nthreads=ncores)
     per = (time.perf_counter() - t0) / 2
-    n = int(max(3, min(200, budget_s / max(per, 1e-6))))
+    n = int(max(3, min(400, budget_s / max(per, 1e-6))))
     t0 = time.perf_counter()
     ref.step(np.full(n, U_MEAN), nthreads=ncores)
     dt = time.perf_counter() - t0
-    # One-thread row: the reference's own loop is single-threaded per rank.
     t0 = time.perf_counter()
     ref.step(np.full(1, U_MEAN), nthreads=1)
     per1 = time.perf_counter() - t0
-    n1 = int(max(1, min(20, 0.25 * budget_s / max(per1, 1e-6))))
+    n1 = int(max(1, min(40, 0.25 * budget_s / max(per1, 1e-6))))
     t0 = time.perf_counter()
     ref.step(np.full(n1, U_MEAN), nthreads=1)
     dt1 = time.perf_counter() - t0
     return {"value": round(geom.n * n / dt / 1e6, 3), "unit": "MLUPS", "cores": ncores, "kind": "port",
-            "sample": f"{n} steps of the same {geom.n}-node bifurcation, oracle FP64 restatement of the "
-                      f"reference step (bounce-back walls, NEE inlet/outlet), OpenMP on {ncores} host threads",
-            "single_thread": {"value": round(geom.n * n1 / dt1 / 1e6, 3), "steps": n1}}
+            "sample": f"{n} steps ({dt:.1f} s) of the same {geom.n}-node bifurcation: the oracle's FP64 restatement "
+                      f"of the reference step with the reference rules (Zou-He inlet at U {U_MEAN}, outlet copy, "
+                      f"wall copy, WSS every step), OpenMP on {ncores} host threads",
+            "single_thread": {"value": round(geom.n * n1 / dt1 / 1e6, 3), "steps": n1}, "host": host}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "columns_step_bifurcation_f32_traffic.md")
+LARGE_PROFILE = os.path.join(ROOT, "profiles", "columns_step_tortuous_vessel_large_share.md")
 
 
-def profiled_traffic():
-    """HBM bytes per FP32 step launch measured by the two rocprofv3 counter
-    passes (FETCH_SIZE, WRITE_SIZE, copy-kernel calibrated) recorded in the
-    profile summary; None when the summary is absent."""
+def profiled_traffic(path=TRAFFIC_PROFILE):
+    """HBM bytes per step launch measured by the two rocprofv3 counter passes
+    (FETCH_SIZE, WRITE_SIZE, copy-kernel calibrated) recorded in a profile
+    summary; None when the summary is absent."""
     try:
-        with open(TRAFFIC_PROFILE) as fh:
+        with open(path) as fh:
             for line in fh:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -379,6 +454,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="f32,f64,f16")
+    ap.add_argument("--precision-steps", type=int, default=PRECISION_STEPS,
+                    help="horizon of the FP32/FP16-vs-FP64 comparison (the workload's 8000 steps)")
+    ap.add_argument("--no-reference-family", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-stenosis", action="store_true")
     ap.add_argument("--stenosis-steps", type=int, default=500)
@@ -399,26 +477,40 @@ def main():
         geom, basis, info = build_geometry(wd)
     N = geom.n
     precs = {"f32": lbm.PREC_F32, "f64": lbm.PREC_F64, "f16": lbm.PREC_F16}
-    runs = {}
-    for name in args.precisions.split(","):
-        s, wall, ev_ms, step_ms, flush_ms = run_precision(geom, basis, precs[name], args.steps, args.warmup)
-        runs[name] = dict(solver=s, wall=wall, ev_ms=ev_ms, step_ms=step_ms, flush_ms=flush_ms,
-                          mlups=N * args.steps / wall / 1e6)
-        print(f"# {name}: {runs[name]['mlups']:.0f} MLUPS (wall {wall:.3f}s, events {ev_ms:.1f}ms)",
-              file=sys.stderr, flush=True)
+    names = args.precisions.split(",")
+    if "f32" not in names:
+        names = ["f32"] + names
+
+    # Precision study over the workload's own horizon (8000 steps from rest),
+    # whatever --steps says. Each run is one batch with HIP-event timing, so
+    # the FP32 run is also the sustained measurement of the step kernel.
+    study = {}
+    for name in names:
+        s, wall, ev_ms, step_ms, flush_ms = run_precision(geom, basis, precs[name], args.precision_steps, 0)
+        study[name] = dict(solver=s, wall=wall, step_ms=step_ms, flush_ms=flush_ms,
+                           mlups=N * args.precision_steps / wall / 1e6)
+        print(f"# {name}: {args.precision_steps} steps, {study[name]['mlups']:.0f} MLUPS, step kernel "
+              f"{step_ms * 1e3 / args.precision_steps:.2f} us", file=sys.stderr, flush=True)
+
+    # The timed region of the bench contract: W warm-up steps, K timed steps.
+    s, wall, ev_ms, step_ms, flush_ms = run_precision(geom, basis, lbm.PREC_F32, args.steps, args.warmup)
+    sinfo = s.info()
+    s.close()
+    value = N * args.steps / wall / 1e6
+    print(f"# timed region: {args.steps} FP32 steps, {value:.0f} MLUPS", file=sys.stderr, flush=True)
 
     copy_gbps = lbm.copy_probe(1 << 30, iters=20)
     print(f"# copy probe: {copy_gbps:.0f} GB/s (1 GiB read + write)", file=sys.stderr, flush=True)
-    head = runs["f32"]
-    sinfo = head["solver"].info()
     bytes_per_launch = N * sinfo.bytes_per_update
     # The dominant kernel's own average launch time: the batch's HIP-event
     # time less the event-timed stress-flush launches (one per 8 steps).
-    per_launch_s = head["step_ms"] / 1e3 / args.steps
+    sus = study["f32"]
+    per_launch_s = sus["step_ms"] / 1e3 / args.precision_steps
+    burst_launch_s = step_ms / 1e3 / args.steps
     achieved = bytes_per_launch / per_launch_s / 1e9
     out = {
-        "metric": METRIC, "value": round(head["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(head["wall"] * 1e3 / args.steps, 5),
+        "metric": METRIC, "value": round(value, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "precision-study Y-bifurcation (parent radius 24, daughters 18 at 35 degrees), "
@@ -429,27 +521,42 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": profiled_traffic(),
                      "bytes_per_launch_algorithmic": bytes_per_launch,
-                     "launch_us": round(per_launch_s * 1e6, 3),
+                     "launch_us": round(per_launch_s * 1e6, 3), "sustained_steps": args.precision_steps,
                      "kernel": "columns_kernel_occ (FP32 step; the two cache-hint variants alternate)",
-                     "flush_us_per_step": round(head["flush_ms"] * 1e3 / args.steps, 3),
+                     "flush_us_per_step": round(sus["flush_ms"] * 1e3 / args.precision_steps, 3),
+                     "burst": {"steps": args.steps, "launch_us": round(burst_launch_s * 1e6, 3),
+                               "frac": round(bytes_per_launch / burst_launch_s / 1e9 / PEAK_HBM_GBS, 4)},
                      "copy_probe_gbps": round(copy_gbps, 1)},
+        "precision_steps": args.precision_steps,
+        "f32_sustained_mlups": round(sus["mlups"], 1),
     }
-    if "f64" in runs:
-        out["fp64_mlups"] = round(runs["f64"]["mlups"], 1)
-        out["fp32_over_fp64"] = round(head["mlups"] / runs["f64"]["mlups"], 3)
-        ref = runs["f64"]["solver"]
+    if "f64" in study:
+        out["fp64_mlups"] = round(study["f64"]["mlups"], 1)
+        out["fp32_over_fp64"] = round(sus["mlups"] / study["f64"]["mlups"], 3)
+        ref = study["f64"]["solver"]
         w64, _, m64 = ref.wss()
         _, u64 = ref.macroscopic()
         for name in ("f32", "f16"):
-            if name not in runs:
+            if name not in study:
                 continue
-            w, _, m = runs[name]["solver"].wss()
-            _, u = runs[name]["solver"].macroscopic()
+            w, _, m = study[name]["solver"].wss()
+            _, u = study[name]["solver"].macroscopic()
             out[f"{name}_wss_rel_l2"] = rel_l2(w, w64)
             out[f"{name}_tawss_rel_l2"] = rel_l2(m, m64)
             out[f"{name}_velocity_rel_l2"] = rel_l2(u, u64)
-    if "f16" in runs:
-        out["fp16_mlups"] = round(runs["f16"]["mlups"], 1)
+    if "f16" in study:
+        out["fp16_mlups"] = round(study["f16"]["mlups"], 1)
+        f16_launch = study["f16"]["step_ms"] / 1e3 / args.precision_steps
+        out["fp16_roofline_frac"] = round(N * 76.0 / f16_launch / 1e9 / PEAK_HBM_GBS, 4)
+    if "f64" in study:
+        f64_launch = study["f64"]["step_ms"] / 1e3 / args.precision_steps
+        out["fp64_roofline_frac"] = round(N * 304.0 / f64_launch / 1e9 / PEAK_HBM_GBS, 4)
+    for v in study.values():
+        v["solver"].close()
+    if not args.no_reference_family:
+        out["reference_family"] = run_reference_family(geom)
+        print(f"# reference family (FP64, reference rules): {out['reference_family']['mlups']:.0f} MLUPS, "
+              f"roofline {out['reference_family']['roofline_frac']:.3f}", file=sys.stderr, flush=True)
     out["adjoint_over_forward"] = None
     if not args.no_adjoint:
         adj = run_adjoint(geom, info, args.adjoint_steps)
@@ -462,7 +569,13 @@ def main():
         print(f"# D3Q27 stenosis: f32 {out['d3q27_stenosis']['f32_mlups']:.0f} MLUPS, "
               f"f64 {out['d3q27_stenosis']['f64_mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     if not args.no_large_share:
-        out["large_share"] = large_share_child(args.large_share_steps)
+        ls = large_share_child(args.large_share_steps)
+        if "nodes" in ls:
+            # The state (45 GB) is far beyond the Infinity Cache: this is the
+            # DRAM-bound figure of the same FP32 step, with its counted traffic.
+            ls["traffic"] = profiled_traffic(LARGE_PROFILE)
+            ls["bytes_per_launch_algorithmic"] = ls["nodes"] * 152
+        out["large_share"] = ls
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(geom)
     print(json.dumps(out))

@@ -63,6 +63,11 @@ struct oracle_state {
   int32_t* up;      /* N: node behind an inlet/outlet along its normal, or -1 */
   int32_t* inlets;  /* inlet nodes, creation order */
   int64_t n_in;
+  int32_t* outs;    /* outlet nodes, outlet block order then creation order */
+  int64_t n_out;
+  int outs_serial;  /* some outlet node copies from an outlet node: keep the sequential order */
+  int32_t* walls;   /* wall nodes, creation order */
+  int64_t n_wall;
   double* prof;     /* inlet profile factor per inlet node */
   int32_t* nw;
   double* nrm;
@@ -146,6 +151,18 @@ oracle_state* oracle_create(const oracle_setup* s) {
   st->n_in = 0;
   for (int64_t n = 0; n < N; ++n)
     if (st->kind[n] == 2) st->inlets[st->n_in++] = (int32_t)n;
+  st->outs = (int32_t*)malloc(sizeof(int32_t) * (N + 1));
+  st->n_out = 0;
+  for (int k = 0; k < s->n_outlets; ++k)
+    for (int64_t n = 0; n < N; ++n)
+      if (st->kind[n] == 3 && st->outlet_id[n] == k) st->outs[st->n_out++] = (int32_t)n;
+  st->outs_serial = 0;
+  for (int64_t i = 0; i < st->n_out; ++i)
+    if (st->up[st->outs[i]] >= 0 && st->kind[st->up[st->outs[i]]] == 3) st->outs_serial = 1;
+  st->walls = (int32_t*)malloc(sizeof(int32_t) * (N + 1));
+  st->n_wall = 0;
+  for (int64_t n = 0; n < N; ++n)
+    if (st->kind[n] == 1) st->walls[st->n_wall++] = (int32_t)n;
   st->prof = (double*)malloc(sizeof(double) * (st->n_in + 1));
   for (int64_t i = 0; i < st->n_in; ++i) st->prof[i] = s->inlet_profile ? s->inlet_profile[i] : 1.0;
   st->n_nw = s->n_nearwall;
@@ -158,7 +175,7 @@ oracle_state* oracle_create(const oracle_setup* s) {
   st->f = (double*)malloc(sizeof(double) * N * Q);
   st->fs = (double*)malloc(sizeof(double) * N * Q);
   st->feq = (double*)malloc(sizeof(double) * N * Q);
-  st->tmp = (double*)malloc(sizeof(double) * N * Q);
+  st->tmp = (double*)malloc(sizeof(double) * Q * (st->n_in + st->n_out + 1));
   st->rho = (double*)malloc(sizeof(double) * N);
   st->u = (double*)malloc(sizeof(double) * 3 * N);
   st->wss = (double*)malloc(sizeof(double) * (st->n_nw + 1));
@@ -171,7 +188,7 @@ oracle_state* oracle_create(const oracle_setup* s) {
 void oracle_destroy(oracle_state* st) {
   if (!st) return;
   free(st->ijk); free(st->kind); free(st->outlet_id); free(st->outlet_normals); free(st->nb); free(st->up);
-  free(st->inlets); free(st->prof); free(st->nw); free(st->nrm); free(st->f); free(st->fs); free(st->feq);
+  free(st->inlets); free(st->outs); free(st->walls); free(st->prof); free(st->nw); free(st->nrm); free(st->f); free(st->fs); free(st->feq);
   free(st->tmp); free(st->rho); free(st->u); free(st->wss); free(st->acc); free(st->accm);
   free(st);
 }
@@ -436,56 +453,76 @@ static void one_step(oracle_state* st, double U, int nthreads) {
       if (m >= 0 && (!ref_walls || v != 0.0)) st->f[(int64_t)m * Q + q] = v;
     }
   /* Half-way bounce-back (Node::BounceBack, Node.cpp:144-148) on every link
-   * whose push neighbour is missing: f[opp](x) = f*[q](x). */
+   * whose push neighbour is missing: f[opp](x) = f*[q](x). Each node writes
+   * only its own populations, so every boundary loop below runs its nodes in
+   * parallel; the sequential order only matters for an outlet node that
+   * copies from another outlet node (outs_serial). */
   if (!ref_walls) {
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
     for (int64_t n = 0; n < N; ++n)
       for (int q = 0; q < Q; ++q)
         if (st->nb[n * Q + q] < 0) st->f[n * Q + st->opp[q]] = st->fs[n * Q + q];
   }
   /* 3-4. Inlet (Simulation.cpp:558-565). */
   if (st->s.inlet_mode == 0) {
-    for (int64_t i = 0; i < st->n_in; ++i) oracle_zouhe(st->f + (int64_t)st->inlets[i] * Q, U * st->prof[i], st->s.inlet_normal);
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
+    for (int64_t i = 0; i < st->n_in; ++i)
+      oracle_zouhe(st->f + (int64_t)st->inlets[i] * Q, U * st->prof[i], st->s.inlet_normal);
   } else if (st->s.inlet_mode == 1) {
-    memcpy(st->tmp, st->f, sizeof(double) * N * Q);
+    /* Every inlet node reads its upstream node as streamed, before any
+     * inlet node is rewritten: copy those first. */
+    for (int64_t i = 0; i < st->n_in; ++i) {
+      const int32_t u = st->up[st->inlets[i]];
+      if (u >= 0) memcpy(st->tmp + i * Q, st->f + (int64_t)u * Q, sizeof(double) * Q);
+    }
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
     for (int64_t i = 0; i < st->n_in; ++i) {
       const int64_t n = st->inlets[i];
       if (st->up[n] < 0) continue;
       const double Ui = U * st->prof[i];
       const double ub[3] = {-st->s.inlet_normal[0] * Ui, -st->s.inlet_normal[1] * Ui, -st->s.inlet_normal[2] * Ui};
-      nee(st, st->tmp + (int64_t)st->up[n] * Q, -1.0, ub, 0, st->f + n * Q);
+      nee(st, st->tmp + i * Q, -1.0, ub, 0, st->f + n * Q);
     }
   }
   /* 5. Outlet (Simulation.cpp:650-741): outlet blocks in order, nodes in
    *    creation order, every q: f[opp q](x) = f[opp q](behind). */
   if (st->s.outlet_mode == 0) {
-    for (int k = 0; k < st->s.n_outlets; ++k)
-      for (int64_t n = 0; n < N; ++n) {
-        if (st->kind[n] != 3 || st->outlet_id[n] != k) continue;
-        const int32_t u = st->up[n];
-        if (u < 0) continue;
-        for (int q = 0; q < Q; ++q) {
-          const int o = st->opp[q];
-          st->f[n * Q + o] = st->f[(int64_t)u * Q + o];
-        }
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1 && !st->outs_serial)
+    for (int64_t i = 0; i < st->n_out; ++i) {
+      const int64_t n = st->outs[i];
+      const int32_t u = st->up[n];
+      if (u < 0) continue;
+      for (int q = 0; q < Q; ++q) {
+        const int o = st->opp[q];
+        st->f[n * Q + o] = st->f[(int64_t)u * Q + o];
       }
+    }
   } else if (st->s.outlet_mode == 1) {
-    memcpy(st->tmp, st->f, sizeof(double) * N * Q);
-    for (int64_t n = 0; n < N; ++n) {
-      if (st->kind[n] != 3 || st->up[n] < 0) continue;
+    double* t = st->tmp + st->n_in * Q;
+    for (int64_t i = 0; i < st->n_out; ++i) {
+      const int32_t u = st->up[st->outs[i]];
+      if (u >= 0) memcpy(t + i * Q, st->f + (int64_t)u * Q, sizeof(double) * Q);
+    }
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
+    for (int64_t i = 0; i < st->n_out; ++i) {
+      const int64_t n = st->outs[i];
+      if (st->up[n] < 0) continue;
       const double z[3] = {0, 0, 0};
-      nee(st, st->tmp + (int64_t)st->up[n] * Q, st->s.outlet_rho, z, 1, st->f + n * Q);
+      nee(st, t + i * Q, st->s.outlet_rho, z, 1, st->f + n * Q);
     }
   }
   /* 6. Walls (Simulation.cpp:746-763): q in order, f[opp] = f[q] where the
    *    push neighbour is missing. */
   if (ref_walls) {
-    for (int64_t n = 0; n < N; ++n) {
-      if (st->kind[n] != 1) continue;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
+    for (int64_t i = 0; i < st->n_wall; ++i) {
+      const int64_t n = st->walls[i];
       for (int q = 0; q < Q; ++q)
         if (st->nb[n * Q + q] < 0) st->f[n * Q + st->opp[q]] = st->f[n * Q + q];
     }
   }
   /* 7. Wall shear stress (Simulation.cpp:568). */
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
   for (int64_t i = 0; i < st->n_nw; ++i) wss_node(st, i);
 }
 

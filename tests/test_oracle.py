@@ -174,3 +174,19 @@ def test_oracle_runs_the_tube_without_nan(tube_geometry):
     inl = kind == lbm.KIND_INLET
     uz = (f[inl] @ c[:, 2]) / rho[inl]
     assert np.median(uz) == pytest.approx(0.05, rel=1e-9)
+
+
+@pytest.mark.parametrize("modes", [(0, 0, 0), (1, 1, 1)], ids=["reference rules", "bounce-back + NEE"])
+def test_oracle_threads_do_not_change_results(tube_geometry, modes):
+    # Every per-node loop of the oracle step writes only its own node, so the
+    # threaded step must equal the serial one bit for bit.
+    wall, inlet, outlet = modes
+    out = []
+    for nthreads in (1, 8):
+        ref = oracle.Reference.from_geometry(tube_geometry, tau=0.8, dt=1e-5, wall_mode=wall, inlet_mode=inlet,
+                                             outlet_mode=outlet)
+        ref.step(np.full(30, 0.05), nthreads=nthreads)
+        out.append((ref.f(), ref.wss()))
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)

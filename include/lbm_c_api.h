@@ -290,6 +290,20 @@ typedef struct {
 /* Wavefront-reduction kernel over the current post-collision state. */
 int lbm_reduce(lbm_solver* s, lbm_stats* out);
 
+/* lbm_reduce without device memory of the solver's own: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc)
+ * and do not want the reduction tied to a workspace sized at lbm_create. The
+ * partial sums go to one static device array of 64 MiB per GPU (held from
+ * the library's load on that GPU until the process ends); the call allocates
+ * nothing and gives the same values as lbm_reduce, bit for bit. A solver
+ * whose reduction needs more than the array (about 4 * N / 256 + 2 * N_nw / 256
+ * doubles, i.e. beyond roughly 500M nodes) gets LBM_ERR_ARG. Every call on
+ * one GPU uses the same array and it keeps the last call's contents: calls
+ * that overlap on one GPU (from two host threads) race and must be
+ * serialised by the caller. */
+int lbm_reduce_noalloc(lbm_solver* s, lbm_stats* out);
+
 /* Run nsteps and report the device time of the step kernels, measured with
  * HIP events on the solver's stream (milliseconds, all steps). */
 int lbm_step_timed(lbm_solver* s, int32_t nsteps, double* ms);

@@ -13,6 +13,7 @@
 #include <cerrno>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,39 @@
 #include "kernels.h"
 
 using namespace lbm;
+
+// Workspace of lbm_reduce_noalloc: the partial sums of its reduction kernels
+// and their folded results, in a static device array instead of memory the
+// solver allocates. Each GPU holds its own copy from the moment the library's
+// code object is loaded on it until the process ends. In host code `scratch`
+// only names the symbol: kernels get the device address that
+// hipGetSymbolAddress returns for the current device (scratch_on).
+constexpr size_t kScratchDoubles = size_t(1) << 23;  // 64 MiB per GPU
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `dev`, which must be the current device;
+// looked up once per device.
+int scratch_on(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(LBM_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(LBM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return LBM_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -1195,21 +1229,30 @@ int lbm_get_macroscopic(lbm_solver* s, double* rho, double* u) {
   return LBM_OK;
 }
 
-int lbm_reduce(lbm_solver* s, lbm_stats* out) {
-  if (!s || !out) return fail(LBM_ERR_ARG, "null argument");
-  HIP_TRY(hipSetDevice(s->prm.device));
+}  // extern "C"
+
+namespace {
+
+// lbm_reduce's body over a given workspace: moments -> 4 partial rows per
+// block at `part`, stress statistics -> 2 rows per block at `wpart`, the six
+// folded values at `res`. Every row and result is written before it is
+// read, so the workspace needs no clearing between calls.
+int reduce_into(lbm_solver* s, double* part, double* wpart, double* res, lbm_stats* out) {
   StepArgs a = args_for(s, s->cur);
-  HIP_TRY(launch_moments(s->Q, s->prec, a, s->partials, s->red_out, s->stream));
+  HIP_TRY(launch_moments(s->Q, s->prec, a, part, res, s->stream));
   if (s->family == 1) {
     int rc = flush_wss(s);
     if (rc) return rc;
   }
-  if (s->family == 1)
-    HIP_TRY(launch_columns_wss_stats(s->cbase_args, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
-  else
-    HIP_TRY(launch_wss_stats(a, s->partials + 4 * (s->npad / BLOCK), s->red_out + 4, s->stream));
-  double h[6];
-  HIP_TRY(hipMemcpyAsync(h, s->red_out, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  double h[6] = {0, 0, 0, 0, 0, 0};
+  const bool any_nw = s->family == 1 ? s->n_nw > 0 : s->n_special_pad > 0;
+  if (any_nw) {
+    if (s->family == 1) HIP_TRY(launch_columns_wss_stats(s->cbase_args, wpart, res + 4, s->stream));
+    else HIP_TRY(launch_wss_stats(a, wpart, res + 4, s->stream));
+    HIP_TRY(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  } else {
+    HIP_TRY(hipMemcpyAsync(h, res, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  }
   HIP_TRY(hipStreamSynchronize(s->stream));
   std::memset(out, 0, sizeof(*out));
   out->mass = h[0];
@@ -1221,6 +1264,35 @@ int lbm_reduce(lbm_solver* s, lbm_stats* out) {
   out->iteration = s->iter;
   out->time = s->total_runtime;
   return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_reduce(lbm_solver* s, lbm_stats* out) {
+  if (!s || !out) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  return reduce_into(s, s->partials, s->partials + 4 * (s->npad / BLOCK), s->red_out, out);
+}
+
+int lbm_reduce_noalloc(lbm_solver* s, lbm_stats* out) {
+  if (!s || !out) return fail(LBM_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->prm.device));
+  double* ws = nullptr;
+  int rc = scratch_on(s->prm.device, &ws);
+  if (rc) return rc;
+  // Same layout as the solver's own workspace: the moments' partial rows
+  // (4 per block), the stress statistics' rows (2 per block), the 6 results.
+  const int64_t off_wss = 4 * (s->npad / BLOCK);
+  const int64_t wss_rows = s->family == 1 ? (s->n_nw + BLOCK - 1) / BLOCK : (s->n_special_pad + BLOCK - 1) / BLOCK;
+  const int64_t off_out = round_up(off_wss + 2 * wss_rows, 32);
+  const int64_t need = off_out + 8;
+  if (need > static_cast<int64_t>(kScratchDoubles))
+    return fail(LBM_ERR_ARG, "lbm_reduce_noalloc: this solver's reduction needs " + std::to_string(need * 8) +
+                                 " bytes of workspace, more than the " + std::to_string(kScratchDoubles * 8) +
+                                 " bytes of its static device array (lbm_reduce has no such limit)");
+  return reduce_into(s, ws, ws + off_wss, ws + off_out, out);
 }
 
 int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const lbm_objective* obj,

@@ -119,6 +119,7 @@ SIGNATURES = [
     ("lbm_get_osi_tawss", [_P, _P, _P], C.c_int),
     ("lbm_get_macroscopic", [_P, _P, _P], C.c_int),
     ("lbm_reduce", [_P, _P], C.c_int),
+    ("lbm_reduce_noalloc", [_P, _P], C.c_int),
     ("lbm_step_timed", [_P, C.c_int32, _P], C.c_int),
     ("lbm_last_batch_time", [_P, _P, _P], C.c_int),
     ("lbm_set_kernel_timing", [_P, C.c_int32], C.c_int),

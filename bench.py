@@ -185,13 +185,13 @@ def run_adjoint(geom, info, steps=ADJ_STEPS):
     wave = case["wave"][:steps]
     coef = lambda th: wave[:, None] * th[None, :]  # noqa: E731
     # Objective weights: the stress term scaled to the size of the pressure term.
+    k = min(ADJ_EVERY, steps)
     cal = adjoint_solver(geom, case, lbm.PREC_F32)
-    jw = cal.adjoint_forward(ADJ_EVERY, coef(case["theta"])[:ADJ_EVERY], wss_weight=1.0, dp_weight=0.0,
-                             wss_mask=case["mask"], checkpoint_every=ADJ_EVERY)
+    jw = cal.adjoint_forward(k, coef(case["theta"])[:k], wss_weight=1.0, dp_weight=0.0,
+                             wss_mask=case["mask"], checkpoint_every=k)
     cal.close()
     cal = adjoint_solver(geom, case, lbm.PREC_F32)
-    jp = cal.adjoint_forward(ADJ_EVERY, coef(case["theta"])[:ADJ_EVERY], wss_weight=0.0, dp_weight=1.0,
-                             checkpoint_every=ADJ_EVERY)
+    jp = cal.adjoint_forward(k, coef(case["theta"])[:k], wss_weight=0.0, dp_weight=1.0, checkpoint_every=k)
     cal.close()
     w_wss = abs(jp / jw) if jw > 0 else 1.0
     obj = dict(wss_weight=w_wss, dp_weight=1.0, wss_mask=case["mask"], checkpoint_every=ADJ_EVERY)
@@ -457,6 +457,7 @@ def main():
     ap.add_argument("--precision-steps", type=int, default=PRECISION_STEPS,
                     help="horizon of the FP32/FP16-vs-FP64 comparison (the workload's 8000 steps)")
     ap.add_argument("--no-reference-family", action="store_true")
+    ap.add_argument("--reference-family-steps", type=int, default=REF_FAMILY_STEPS)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-stenosis", action="store_true")
     ap.add_argument("--stenosis-steps", type=int, default=500)
@@ -554,7 +555,7 @@ def main():
     for v in study.values():
         v["solver"].close()
     if not args.no_reference_family:
-        out["reference_family"] = run_reference_family(geom)
+        out["reference_family"] = run_reference_family(geom, args.reference_family_steps)
         print(f"# reference family (FP64, reference rules): {out['reference_family']['mlups']:.0f} MLUPS, "
               f"roofline {out['reference_family']['roofline_frac']:.3f}", file=sys.stderr, flush=True)
     out["adjoint_over_forward"] = None

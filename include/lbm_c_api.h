@@ -142,6 +142,40 @@ int lbm_geometry_from_labels(const uint8_t* fluid, int32_t nx, int32_t ny, int32
                              const double* tris, int64_t ntris,
                              const lbm_geometry_options* opts, lbm_geometry** out);
 
+/* The node tables of a reference SubDomain, handed over as they stand
+ * (Subdomain.h:66-118, Node.h:53-70), so that the reference's own set-up can
+ * drive the GPU path without a second voxelisation. Every array is borrowed
+ * for the call and copied. */
+typedef struct {
+  int32_t q;                     /* 19 or 27 (_nLatNodes) */
+  int64_t n_nodes;               /* N = Involved_Lat_.size() */
+  int32_t grid[3];               /* Geometry::pointsBox extents (nx+1, ny+1, nz+1) */
+  double origin[3];              /* coordinates of grid index (0,0,0): the bounding-box minimum */
+  double spacing;                /* Geometry::spacing, the lattice pitch (twice grid_size) */
+  const int32_t* ijk;            /* 3N: Node::x_position, y_position, z_position, Involved_Lat_ order */
+  const uint8_t* kind;           /* N: LBM_KIND_* (Node::NodeType_ and BoundaryType_) */
+  const int32_t* outlet_id;      /* N: the OutletBoundaryNodes_ block holding the node, else -1 (NULL: no outlets) */
+  const int32_t* nb;             /* Q*N: Node::m_neighbours[q] as a node index, nb[q*N + n], -1 for none */
+  int32_t has_inlet_normal;      /* InletBoundaryNode_.second is set */
+  double inlet_normal[3];        /* InletBoundaryNode_.second */
+  int32_t n_outlets;             /* OutletBoundaryNodes_.size() */
+  const double* outlet_normals;  /* 3 per outlet block: OutletBoundaryNodes_[k].second */
+  int64_t n_nearwall;            /* nextToTheWall.size() */
+  const int32_t* nearwall;       /* plain fluid node indices, in the order results are to be reported */
+  const double* normals;         /* 3 per near-wall node: Node::normal */
+} lbm_tables;
+
+/* Replaces the Geometry + SubDomain constructors with tables the caller built
+ * (the reference's own SubDomain, for one): checks them (sizes, kinds, outlet
+ * blocks, node positions inside the grid and unique, every neighbour entry at
+ * x + c_q, near-wall nodes plain fluid) and derives what the solver needs:
+ * the creation-order node lists, and the node behind each inlet/outlet node
+ * along its block normal (the +x,-x,+y,-y,+z,-z else-if of
+ * Simulation::ApplyOutletBc, Simulation.cpp:650-741, read from nb). The
+ * near-wall order given is the order lbm_get_wss and the output files use.
+ * No triangles: lbm_geometry_get_triangles returns none. */
+int lbm_geometry_from_tables(const lbm_tables* t, lbm_geometry** out);
+
 typedef struct {
   int64_t n_nodes, n_fluid, n_wall, n_inlet, n_outlet, n_nearwall, n_boundary_points;
   int64_t n_triangles, upstream_out_of_box;
@@ -286,6 +320,14 @@ typedef struct {
   int64_t iteration;
   double time;             /* reference total_runtime */
 } lbm_stats;
+
+/* Replaces Boundary::ApplyBc(double& vel, std::vector<double>& normal)
+ * (Boundary.h:61, Boundary.cpp:111-289) for `count` independent D3Q19
+ * population vectors f[i*19 + q] (FP64, in place) with inlet speeds vel[i]
+ * and one normal (InletBoundaryNode_.second), evaluated on HIP device
+ * `device` by the device code the fused step kernel runs for the reference
+ * inlet rule. */
+int lbm_zouhe(int32_t device, int64_t count, double* f, const double* vel, const double normal[3]);
 
 /* Wavefront-reduction kernel over the current post-collision state. */
 int lbm_reduce(lbm_solver* s, lbm_stats* out);

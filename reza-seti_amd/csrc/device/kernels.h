@@ -88,6 +88,10 @@ hipError_t launch_wss_stats(const StepArgs& a, double* partials, double* out2, h
 
 // Deterministic fold of nblk partial rows of nv doubles (nv = 4: sums;
 // nv = 2 with maxlast: sum, max) into out.
+// Boundary::ApplyBc (Boundary.cpp:111-289) on `count` D3Q19 FP64 vectors
+// f[i * 19 + q], in place, inlet speed vel[i], one normal.
+hipError_t launch_zouhe_batch(double* f, const double* vel, int64_t count, const double normal[3], hipStream_t st);
+
 hipError_t launch_finish(const double* partials, int32_t nblk, int nv, bool maxlast, double* out, hipStream_t st);
 
 }  // namespace lbm

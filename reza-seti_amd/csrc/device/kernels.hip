@@ -416,7 +416,31 @@ hipError_t moments_one(const StepArgs& a, double* partials, double* out4, hipStr
   return hipGetLastError();
 }
 
+struct Normal3 {
+  double n[3];
+};
+
+// Boundary::ApplyBc on independent population vectors (D3Q19, FP64).
+__global__ __launch_bounds__(BLOCK) void zouhe_batch_kernel(double* f, const double* vel, int64_t count, Normal3 nrm) {
+  const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i >= count) return;
+  double m[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) m[q] = f[i * 19 + q];
+  zouhe_ref<double, false>(m, vel[i], nrm.n);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) f[i * 19 + q] = m[q];
+}
+
 }  // namespace
+
+hipError_t launch_zouhe_batch(double* f, const double* vel, int64_t count, const double normal[3], hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  Normal3 n{{normal[0], normal[1], normal[2]}};
+  const unsigned grid = static_cast<unsigned>((count + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(zouhe_batch_kernel, dim3(grid), dim3(BLOCK), 0, st, f, vel, count, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_finish(const double* partials, int32_t nblk, int nv, bool maxlast, double* out, hipStream_t st) {
   if (nv == 4 && !maxlast) hipLaunchKernelGGL((finish_kernel<4, false>), dim3(1), dim3(BLOCK), 0, st, partials, nblk, out);

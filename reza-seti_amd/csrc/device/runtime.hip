@@ -1270,6 +1270,27 @@ int reduce_into(lbm_solver* s, double* part, double* wpart, double* res, lbm_sta
 
 extern "C" {
 
+int lbm_zouhe(int32_t device, int64_t count, double* f, const double* vel, const double normal[3]) {
+  if (count < 0 || !normal || (count > 0 && (!f || !vel))) return fail(LBM_ERR_ARG, "invalid argument");
+  if (count == 0) return LBM_OK;
+  HIP_TRY(hipSetDevice(device));
+  double *df = nullptr, *dv = nullptr;
+  struct Free {
+    double** p[2];
+    ~Free() {
+      for (double** q : p)
+        if (*q) (void)hipFree(*q);
+    }
+  } guard{{&df, &dv}};
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&df), static_cast<size_t>(count) * 19 * sizeof(double)));
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&dv), static_cast<size_t>(count) * sizeof(double)));
+  HIP_TRY(hipMemcpy(df, f, static_cast<size_t>(count) * 19 * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dv, vel, static_cast<size_t>(count) * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(launch_zouhe_batch(df, dv, count, normal, nullptr));
+  HIP_TRY(hipMemcpy(f, df, static_cast<size_t>(count) * 19 * sizeof(double), hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
 int lbm_reduce(lbm_solver* s, lbm_stats* out) {
   if (!s || !out) return fail(LBM_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->prm.device));

@@ -250,7 +250,47 @@ int lbm_geometry_get_info(const lbm_geometry* g, lbm_geometry_info* o) {
 
 int lbm_geometry_get_labels(const lbm_geometry* g, uint8_t* fluid) {
   if (!g || !fluid) return fail(LBM_ERR_ARG, "null argument");
+  if (static_cast<int64_t>(g->t.fluid.size()) != g->t.grid.npoints())
+    return fail(LBM_ERR_UNSUPPORTED, "no grid labels kept for this geometry (tables over more than 2^32 grid points)");
   std::memcpy(fluid, g->t.fluid.data(), g->t.fluid.size());
+  return LBM_OK;
+}
+
+int lbm_geometry_from_tables(const lbm_tables* in, lbm_geometry** out) {
+  if (!in || !out) return fail(LBM_ERR_ARG, "null argument");
+  *out = nullptr;
+  TableInput ti;
+  ti.Q = in->q;
+  ti.N = in->n_nodes;
+  for (int a = 0; a < 3; ++a) {
+    ti.dims[a] = in->grid[a];
+    ti.origin[a] = in->origin[a];
+    ti.inlet_normal[a] = in->inlet_normal[a];
+  }
+  ti.spacing = in->spacing;
+  ti.ijk = in->ijk;
+  ti.kind = in->kind;
+  ti.outlet_id = in->outlet_id;
+  ti.nb = in->nb;
+  ti.has_inlet_normal = in->has_inlet_normal != 0;
+  ti.n_outlets = in->n_outlets;
+  ti.outlet_normals = in->outlet_normals;
+  ti.n_nearwall = in->n_nearwall;
+  ti.nearwall = in->nearwall;
+  ti.normals = in->normals;
+  auto* g = new lbm_geometry();
+  std::string err;
+  bool ok = false;
+  try {
+    ok = build_from_tables(ti, g->t, &err);
+  } catch (const std::exception& e) {
+    err = e.what();
+  }
+  if (!ok) {
+    delete g;
+    return fail(LBM_ERR_ARG, err);
+  }
+  *out = g;
   return LBM_OK;
 }
 

@@ -525,6 +525,126 @@ bool build_from_labels(const GridSpec& g, const std::vector<uint8_t>& fluid,
   return true;
 }
 
+bool build_from_tables(const TableInput& in, Tables& t, std::string* err) {
+  auto bad = [&](const std::string& m) {
+    if (err) *err = "lbm_tables: " + m;
+    return false;
+  };
+  if (in.Q != 19 && in.Q != 27) return bad("q must be 19 or 27");
+  if (in.N <= 0 || in.N >= (int64_t(1) << 31) - 1) return bad("n_nodes must be in [1, 2^31 - 1)");
+  if (!in.ijk || !in.kind || !in.nb) return bad("ijk, kind and nb are required");
+  if (in.dims[0] <= 0 || in.dims[1] <= 0 || in.dims[2] <= 0) return bad("grid extents must be positive");
+  if (!(in.spacing > 2e-6)) return bad("spacing must exceed 2e-6");
+  if (in.n_outlets < 0 || (in.n_outlets > 0 && !in.outlet_normals)) return bad("outlet normals missing");
+  if (in.n_nearwall < 0 || (in.n_nearwall > 0 && (!in.nearwall || !in.normals))) return bad("near-wall arrays missing");
+  const LatticeView L = lattice_view(in.Q);
+  const int Q = L.Q;
+  const int64_t N = in.N;
+  t = Tables();
+  t.Q = Q;
+  t.N = N;
+  t.grid.NX = in.dims[0];
+  t.grid.NY = in.dims[1];
+  t.grid.NZ = in.dims[2];
+  t.grid.spacing = in.spacing;
+  t.grid.bbox = {in.origin[0], in.origin[0] + (in.dims[0] - 1) * in.spacing,
+                 in.origin[1], in.origin[1] + (in.dims[1] - 1) * in.spacing,
+                 in.origin[2], in.origin[2] + (in.dims[2] - 1) * in.spacing};
+  t.ijk.assign(in.ijk, in.ijk + 3 * N);
+  t.kind.assign(in.kind, in.kind + N);
+  t.outlet_id.assign(N, -1);
+  t.outlet_nodes.assign(in.n_outlets, {});
+  for (int k = 0; k < in.n_outlets; ++k)
+    t.outlet_normals.push_back({in.outlet_normals[3 * k], in.outlet_normals[3 * k + 1], in.outlet_normals[3 * k + 2]});
+  t.has_inlet_normal = in.has_inlet_normal;
+  if (in.has_inlet_normal) t.inlet_normal = {in.inlet_normal[0], in.inlet_normal[1], in.inlet_normal[2]};
+
+  // Positions inside the grid and unique (one node per grid point).
+  std::vector<int64_t> key(N);
+  for (int64_t n = 0; n < N; ++n) {
+    const int32_t* c = &t.ijk[3 * n];
+    if (c[0] < 0 || c[0] >= in.dims[0] || c[1] < 0 || c[1] >= in.dims[1] || c[2] < 0 || c[2] >= in.dims[2])
+      return bad("node " + std::to_string(n) + " lies outside the grid");
+    key[n] = t.grid.gidx(c[0], c[1], c[2]);
+  }
+  {
+    std::vector<int64_t> sorted(key);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return bad("two nodes share a grid point");
+  }
+  // Kinds and outlet blocks; creation-order lists (node order is creation order).
+  for (int64_t n = 0; n < N; ++n) {
+    const uint8_t k = t.kind[n];
+    const int32_t oid = in.outlet_id ? in.outlet_id[n] : -1;
+    if (k > KIND_OUTLET) return bad("node " + std::to_string(n) + " has an unknown kind");
+    if ((k == KIND_OUTLET) != (oid >= 0)) return bad("node " + std::to_string(n) + ": outlet block and kind disagree");
+    if (oid >= in.n_outlets) return bad("node " + std::to_string(n) + ": outlet block out of range");
+    if (k == KIND_INLET) t.inlet_nodes.push_back(static_cast<int32_t>(n));
+    if (k == KIND_WALL) t.wall_nodes.push_back(static_cast<int32_t>(n));
+    if (k == KIND_OUTLET) {
+      t.outlet_id[n] = oid;
+      t.outlet_nodes[oid].push_back(static_cast<int32_t>(n));
+    }
+    if (k != KIND_FLUID) t.n_boundary_points++;
+  }
+  // Neighbour entries: in range and at x + c_q (no wrap: the reference has
+  // no periodic axes).
+  t.nb.assign(in.nb, in.nb + static_cast<size_t>(Q) * N);
+  for (int q = 0; q < Q; ++q)
+    for (int64_t n = 0; n < N; ++n) {
+      const int32_t m = t.nb[static_cast<size_t>(q) * N + n];
+      if (m < 0) continue;
+      if (m >= N) return bad("neighbour entry out of range");
+      const int32_t* a = &t.ijk[3 * n];
+      const int32_t* b = &t.ijk[3 * int64_t(m)];
+      if (b[0] != a[0] + L.cx[q] || b[1] != a[1] + L.cy[q] || b[2] != a[2] + L.cz[q])
+        return bad("neighbour " + std::to_string(q) + " of node " + std::to_string(n) + " is not at x + c_q");
+    }
+  // The node behind an inlet/outlet node: its neighbour along -n_hat, taken
+  // in the reference's else-if order of the normal's components.
+  t.bc_nbr.assign(N, -1);
+  auto behind = [&](int32_t n, const std::array<double, 3>& nv) -> int32_t {
+    int d[3] = {0, 0, 0};
+    if (nv[0] > 0) d[0] = -1;
+    else if (nv[0] < 0) d[0] = 1;
+    else if (nv[1] > 0) d[1] = -1;
+    else if (nv[1] < 0) d[1] = 1;
+    else if (nv[2] > 0) d[2] = -1;
+    else if (nv[2] < 0) d[2] = 1;
+    else return -1;
+    const int32_t* c = &t.ijk[3 * int64_t(n)];
+    for (int a = 0; a < 3; ++a)
+      if (c[a] + d[a] < 0 || c[a] + d[a] >= in.dims[a]) {
+        t.upstream_out_of_box++;
+        return -1;
+      }
+    for (int q = 0; q < Q; ++q)
+      if (L.cx[q] == d[0] && L.cy[q] == d[1] && L.cz[q] == d[2]) return t.nb[static_cast<size_t>(q) * N + n];
+    return -1;
+  };
+  for (size_t s = 0; s < t.outlet_nodes.size(); ++s)
+    for (int32_t n : t.outlet_nodes[s]) t.bc_nbr[n] = behind(n, t.outlet_normals[s]);
+  if (t.has_inlet_normal)
+    for (int32_t n : t.inlet_nodes) t.bc_nbr[n] = behind(n, t.inlet_normal);
+  // Near-wall nodes as given.
+  std::vector<uint8_t> seen(N, 0);
+  for (int64_t i = 0; i < in.n_nearwall; ++i) {
+    const int32_t n = in.nearwall[i];
+    if (n < 0 || n >= N) return bad("near-wall node out of range");
+    if (t.kind[n] != KIND_FLUID) return bad("near-wall node " + std::to_string(n) + " is not a plain fluid node");
+    if (seen[n]++) return bad("near-wall node " + std::to_string(n) + " listed twice");
+    t.nearwall.push_back(n);
+  }
+  t.normals.assign(in.normals, in.normals + 3 * in.n_nearwall);
+  // Grid labels from the node positions (Point::Type Fluid), where the grid
+  // is small enough to hold them.
+  if (t.grid.npoints() <= (int64_t(1) << 32)) {
+    t.fluid.assign(static_cast<size_t>(t.grid.npoints()), 0);
+    for (int64_t n = 0; n < N; ++n) t.fluid[key[n]] = 1;
+  }
+  return true;
+}
+
 bool build_from_config(const SimulationParameters& p, const std::string& base_dir, const BuildOptions& o,
                        Tables& t, std::vector<Triangle>* tris_out, std::string* err) {
   std::string path = p.stlFile;

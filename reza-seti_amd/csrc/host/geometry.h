@@ -100,6 +100,28 @@ bool build_from_labels(const GridSpec& g, const std::vector<uint8_t>& fluid,
                        const std::vector<Triangle>& tris, const BuildOptions& o, Tables& t,
                        std::string* err);
 
+// Node tables given directly (a reference SubDomain's), checked; derives the
+// node lists and the inlet/outlet neighbour behind each boundary node.
+struct TableInput {
+  int Q = 19;
+  int64_t N = 0;
+  int dims[3] = {0, 0, 0};
+  double origin[3] = {0, 0, 0};
+  double spacing = 0.0;
+  const int32_t* ijk = nullptr;
+  const uint8_t* kind = nullptr;
+  const int32_t* outlet_id = nullptr;
+  const int32_t* nb = nullptr;
+  bool has_inlet_normal = false;
+  double inlet_normal[3] = {0, 0, 0};
+  int n_outlets = 0;
+  const double* outlet_normals = nullptr;
+  int64_t n_nearwall = 0;
+  const int32_t* nearwall = nullptr;
+  const double* normals = nullptr;
+};
+bool build_from_tables(const TableInput& in, Tables& t, std::string* err);
+
 // Near-wall normal of one point (Subdomain.cpp:519-570), exposed for tests.
 std::array<double, 3> boundary_normal(double x, double y, double z, const std::vector<Triangle>& tris);
 

@@ -73,6 +73,17 @@ class Geometry {
     spacing = info_.spacing;
     GeomSize = {info_.grid[0], info_.grid[1], info_.grid[2]};
   }
+  // The same, from node tables a reference SubDomain already holds
+  // (lbm_geometry_from_tables): no second voxelisation.
+  Geometry(std::shared_ptr<LatticeModel> velSet, SimulationParameters& simParams, const lbm_tables& tables)
+      : velSet_(std::move(velSet)), simParams(simParams) {
+    lbm_geometry* g = nullptr;
+    check(lbm_geometry_from_tables(&tables, &g), "Geometry");
+    handle_.reset(g, lbm_geometry_destroy);
+    check(lbm_geometry_get_info(g, &info_), "Geometry info");
+    spacing = info_.spacing;
+    GeomSize = {info_.grid[0], info_.grid[1], info_.grid[2]};
+  }
   lbm_geometry* handle() const { return handle_.get(); }
   const lbm_geometry_info& info() const { return info_; }
 
@@ -84,6 +95,23 @@ class Geometry {
  private:
   std::shared_ptr<lbm_geometry> handle_;
   lbm_geometry_info info_{};
+};
+
+// Boundary::ApplyBc (Boundary.h:61, Boundary.cpp:111-289) on one node's
+// populations: the reference's Zou-He inlet rule, evaluated by the device
+// code the fused step runs for inlet nodes (lbm_zouhe).
+class Boundary {
+ public:
+  explicit Boundary(int device = 0) : m_distributions(19, 0.0), device_(device) {}
+  void ApplyBc(double& vel, std::vector<double>& normal) {
+    if (normal.size() < 3 || m_distributions.size() != 19)
+      throw std::invalid_argument("ApplyBc: 19 populations and a 3-component normal");
+    check(lbm_zouhe(device_, 1, m_distributions.data(), &vel, normal.data()), "ApplyBc");
+  }
+  std::vector<double> m_distributions;
+
+ private:
+  int device_;
 };
 
 // SubDomain: node lists of one rank (Subdomain.cpp:43-62). On one GPU the

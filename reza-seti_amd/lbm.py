@@ -61,6 +61,14 @@ class GeometryInfo(C.Structure):
                 ("bbox", C.c_double * 6), ("inlet_normal", C.c_double * 3)]
 
 
+class TablesIn(C.Structure):
+    _fields_ = [("q", C.c_int32), ("n_nodes", C.c_int64), ("grid", C.c_int32 * 3), ("origin", C.c_double * 3),
+                ("spacing", C.c_double), ("ijk", C.c_void_p), ("kind", C.c_void_p), ("outlet_id", C.c_void_p),
+                ("nb", C.c_void_p), ("has_inlet_normal", C.c_int32), ("inlet_normal", C.c_double * 3),
+                ("n_outlets", C.c_int32), ("outlet_normals", C.c_void_p), ("n_nearwall", C.c_int64),
+                ("nearwall", C.c_void_p), ("normals", C.c_void_p)]
+
+
 class Params(C.Structure):
     _fields_ = [("precision", C.c_int32), ("wall_mode", C.c_int32), ("inlet_mode", C.c_int32),
                 ("outlet_mode", C.c_int32), ("flow_type", C.c_int32), ("wss_every", C.c_int32),
@@ -95,6 +103,7 @@ SIGNATURES = [
     ("lbm_config_parse", [C.c_char_p, _P], C.c_int),
     ("lbm_geometry_options_default", [_P], None),
     ("lbm_geometry_from_config", [C.c_char_p, _P, _P], C.c_int),
+    ("lbm_geometry_from_tables", [_P, _P], C.c_int),
     ("lbm_geometry_from_labels", [_P, C.c_int32, C.c_int32, C.c_int32, _P, C.c_double, _P, C.c_int32, _P,
                                   C.c_int32, _P, C.c_int64, _P, _P], C.c_int),
     ("lbm_geometry_get_info", [_P, _P], C.c_int),
@@ -119,6 +128,7 @@ SIGNATURES = [
     ("lbm_get_osi_tawss", [_P, _P, _P], C.c_int),
     ("lbm_get_macroscopic", [_P, _P, _P], C.c_int),
     ("lbm_reduce", [_P, _P], C.c_int),
+    ("lbm_zouhe", [C.c_int32, C.c_int64, _P, _P, _P], C.c_int),
     ("lbm_reduce_noalloc", [_P, _P], C.c_int),
     ("lbm_step_timed", [_P, C.c_int32, _P], C.c_int),
     ("lbm_last_batch_time", [_P, _P, _P], C.c_int),
@@ -192,6 +202,15 @@ def lattice(q=19):
 def inlet_velocity(t, period=0.9):
     """Simulation::getInletVelocity: the pulsatile inlet speed lbm_step uses."""
     return lib().lbm_inlet_velocity(float(t), float(period))
+
+
+def zouhe(f, vel, normal, device=0):
+    """Boundary::ApplyBc on D3Q19 population vectors (rows of f), on the GPU."""
+    f = np.array(f, dtype=np.float64, order="C").reshape(-1, 19)
+    v = np.ascontiguousarray(np.broadcast_to(np.asarray(vel, np.float64), (f.shape[0],)))
+    n = np.ascontiguousarray(normal, np.float64)
+    _check(lib().lbm_zouhe(int(device), f.shape[0], _ptr(f), _ptr(v), _ptr(n)))
+    return f
 
 
 def parse_config(path):
@@ -268,6 +287,40 @@ class Geometry:
         _check(lib().lbm_geometry_from_labels(_ptr(fluid), nx, ny, nz, _ptr(org), float(spacing), ins,
                                               len(inlets), outs, len(outlets), _ptr(tri), tri.shape[0],
                                               C.byref(o), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_tables(cls, q, grid, origin, spacing, ijk, kind, nb, outlet_id=None, inlet_normal=None,
+                    outlet_normals=None, nearwall=None, normals=None):
+        """Node tables as a reference SubDomain holds them (lbm_geometry_from_tables)."""
+        keep = [np.ascontiguousarray(ijk, np.int32), np.ascontiguousarray(kind, np.uint8),
+                np.ascontiguousarray(nb, np.int32)]
+        t = TablesIn()
+        t.q = q
+        t.n_nodes = len(keep[1])
+        for a in range(3):
+            t.grid[a] = int(grid[a])
+            t.origin[a] = float(origin[a])
+        t.spacing = float(spacing)
+        t.ijk, t.kind, t.nb = [k.ctypes.data for k in keep]
+        if outlet_id is not None:
+            keep.append(np.ascontiguousarray(outlet_id, np.int32))
+            t.outlet_id = keep[-1].ctypes.data
+        if inlet_normal is not None:
+            t.has_inlet_normal = 1
+            for a in range(3):
+                t.inlet_normal[a] = float(inlet_normal[a])
+        if outlet_normals is not None and len(outlet_normals):
+            keep.append(np.ascontiguousarray(outlet_normals, np.float64).reshape(-1, 3))
+            t.n_outlets = keep[-1].shape[0]
+            t.outlet_normals = keep[-1].ctypes.data
+        if nearwall is not None and len(nearwall):
+            keep.append(np.ascontiguousarray(nearwall, np.int32))
+            keep.append(np.ascontiguousarray(normals, np.float64).reshape(-1, 3))
+            t.n_nearwall = keep[-2].shape[0]
+            t.nearwall, t.normals = keep[-2].ctypes.data, keep[-1].ctypes.data
+        h = C.c_void_p()
+        _check(lib().lbm_geometry_from_tables(C.byref(t), C.byref(h)))
         return cls(h.value)
 
     @property

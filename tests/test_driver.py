@@ -120,3 +120,35 @@ def test_driver_runs_the_pulsatile_loop(tmp_path):
     assert head == "X,Y,Z,WSS,OSI,TAWSS,VelMag"
     np.testing.assert_allclose(raw[:, 4], osi, rtol=1e-5, atol=1e-12)
     np.testing.assert_allclose(raw[:, 5], tawss, rtol=1e-5, atol=0)
+
+
+@pytest.mark.gpu
+def test_driver_runs_the_reference_case(tmp_path):
+    # The reference's own case: its config.txt and JT.stl at grid_size 0.025
+    # (6.6M nodes, one inlet and four outlet blocks, Constant CteVel 0.1,
+    # tau 0.68), through lbm_run, bit for bit against the oracle.
+    import lbm
+    import sys
+
+    from conftest import GOLDEN
+
+    sys.path.insert(0, GOLDEN)
+    import cases
+
+    steps = 300
+    cfg = cases.jt_case(str(tmp_path), 0.025)
+    dump = tmp_path / "f.bin"
+    print("lbm_run on the reference case", flush=True)
+    r = subprocess.run([EXE, cfg, "--steps", str(steps), "--dump", str(dump)], capture_output=True, text=True,
+                       timeout=200)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert f"Iterations: {steps}," in r.stdout
+    geom = lbm.Geometry.from_config(cfg)
+    assert geom.n > 6_000_000 and geom.info.n_triangles == 9712
+    print(f"oracle: {geom.n} nodes, {steps} steps", flush=True)
+    ref = oracle.Reference.from_geometry(geom, tau=0.68, dt=1e-5)
+    ref.step(np.full(steps, 0.1), nthreads=min(16, os.cpu_count() or 1))
+    f = np.fromfile(dump, dtype=np.float64).reshape(geom.n, 19)
+    assert np.isfinite(f).all()
+    assert np.array_equal(f, ref.f())

@@ -289,6 +289,14 @@ int lbm_get_osi_tawss(lbm_solver* s, double* osi, double* tawss);
  * lbm_write_output adds it. */
 int lbm_get_macroscopic(lbm_solver* s, double* rho, double* u);
 
+/* Density and velocity (sum f, sum c f / sum f, FP64 sums) of the
+ * post-collision populations f* that the next step streams (the state
+ * lbm_get_post_collision returns), at `count` given nodes only: a probe of a
+ * state too large to read back whole. Unforced BGK leaves a node's density
+ * and momentum as its collision found them (to rounding), so away from
+ * reference-rule walls these are the density and velocity of f(t). */
+int lbm_get_post_collision_moments(lbm_solver* s, const int32_t* nodes, int64_t count, double* rho, double* u);
+
 /* Simulation::WriteOutput (Simulation.cpp:835-856) with `dir` standing for
  * the working directory: dir/Fluid_Variables/<iter>.vtu
  * (OutputFile::WriteVTK_Collective, OutputFile.cpp:60-231: nodes in z-y-x

@@ -88,6 +88,11 @@ hipError_t launch_wss_stats(const StepArgs& a, double* partials, double* out2, h
 
 // Deterministic fold of nblk partial rows of nv doubles (nv = 4: sums;
 // nv = 2 with maxlast: sum, max) into out.
+// Density and velocity (rho, jx/rho, jy/rho, jz/rho) of a state buffer at
+// `count` slots, 4 FP64 values per slot to out.
+hipError_t launch_moments_at(int Q, int precision, const void* fs, int64_t npad, const int32_t* slots, int64_t count,
+                             double* out, hipStream_t st);
+
 // Boundary::ApplyBc (Boundary.cpp:111-289) on `count` D3Q19 FP64 vectors
 // f[i * 19 + q], in place, inlet speed vel[i], one normal.
 hipError_t launch_zouhe_batch(double* f, const double* vel, int64_t count, const double normal[3], hipStream_t st);

@@ -416,6 +416,42 @@ hipError_t moments_one(const StepArgs& a, double* partials, double* out4, hipStr
   return hipGetLastError();
 }
 
+// Density and velocity (FP64 sums) of the state buffer's populations at
+// the given slots: 4 values per entry.
+template <class L, class S>
+__global__ __launch_bounds__(BLOCK) void moments_at_kernel(const void* fs, int64_t npad, const int32_t* slots,
+                                                           int64_t count, double* out) {
+  using P = Pop<S>;
+  const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i >= count) return;
+  const int64_t p = slots[i];
+  double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    double f = double(P::ld(fs, int64_t(q) * npad + p));
+    if (P::shifted) f += L::w[q];
+    r += f;
+    jx += L::cx[q] * f;
+    jy += L::cy[q] * f;
+    jz += L::cz[q] * f;
+  }
+  out[4 * i] = r;
+  out[4 * i + 1] = jx / r;
+  out[4 * i + 2] = jy / r;
+  out[4 * i + 3] = jz / r;
+}
+
+template <class L>
+hipError_t moments_at_prec(int precision, const void* fs, int64_t npad, const int32_t* slots, int64_t count,
+                           double* out, hipStream_t st) {
+  const unsigned grid = static_cast<unsigned>((count + BLOCK - 1) / BLOCK);
+  if (precision == 0) hipLaunchKernelGGL((moments_at_kernel<L, double>), dim3(grid), dim3(BLOCK), 0, st, fs, npad, slots, count, out);
+  else if (precision == 1) hipLaunchKernelGGL((moments_at_kernel<L, float>), dim3(grid), dim3(BLOCK), 0, st, fs, npad, slots, count, out);
+  else if (precision == 2) hipLaunchKernelGGL((moments_at_kernel<L, __half>), dim3(grid), dim3(BLOCK), 0, st, fs, npad, slots, count, out);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 struct Normal3 {
   double n[3];
 };
@@ -433,6 +469,14 @@ __global__ __launch_bounds__(BLOCK) void zouhe_batch_kernel(double* f, const dou
 }
 
 }  // namespace
+
+hipError_t launch_moments_at(int Q, int precision, const void* fs, int64_t npad, const int32_t* slots, int64_t count,
+                             double* out, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  if (Q == 19) return moments_at_prec<D3Q19>(precision, fs, npad, slots, count, out, st);
+  if (Q == 27) return moments_at_prec<D3Q27>(precision, fs, npad, slots, count, out, st);
+  return hipErrorInvalidValue;
+}
 
 hipError_t launch_zouhe_batch(double* f, const double* vel, int64_t count, const double normal[3], hipStream_t st) {
   if (count <= 0) return hipSuccess;

@@ -1270,6 +1270,40 @@ int reduce_into(lbm_solver* s, double* part, double* wpart, double* res, lbm_sta
 
 extern "C" {
 
+int lbm_get_post_collision_moments(lbm_solver* s, const int32_t* nodes, int64_t count, double* rho, double* u) {
+  if (!s || count < 0 || (count > 0 && !nodes)) return fail(LBM_ERR_ARG, "invalid argument");
+  if (count == 0) return LBM_OK;
+  std::vector<int32_t> slots(static_cast<size_t>(count));
+  for (int64_t i = 0; i < count; ++i) {
+    if (nodes[i] < 0 || nodes[i] >= s->N) return fail(LBM_ERR_ARG, "node index out of range");
+    slots[i] = s->slot_of[nodes[i]];
+  }
+  HIP_TRY(hipSetDevice(s->prm.device));
+  int32_t* dslots = nullptr;
+  double* dout = nullptr;
+  struct Free {
+    int32_t** a;
+    double** b;
+    ~Free() {
+      if (*a) (void)hipFree(*a);
+      if (*b) (void)hipFree(*b);
+    }
+  } guard{&dslots, &dout};
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&dslots), slots.size() * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&dout), static_cast<size_t>(count) * 4 * sizeof(double)));
+  HIP_TRY(hipMemcpyAsync(dslots, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(launch_moments_at(s->Q, s->prec, s->fs[s->cur], s->npad, dslots, count, dout, s->stream));
+  std::vector<double> h(static_cast<size_t>(count) * 4);
+  HIP_TRY(hipMemcpyAsync(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int64_t i = 0; i < count; ++i) {
+    if (rho) rho[i] = h[4 * i];
+    if (u)
+      for (int a = 0; a < 3; ++a) u[3 * i + a] = h[4 * i + 1 + a];
+  }
+  return LBM_OK;
+}
+
 int lbm_zouhe(int32_t device, int64_t count, double* f, const double* vel, const double normal[3]) {
   if (count < 0 || !normal || (count > 0 && (!f || !vel))) return fail(LBM_ERR_ARG, "invalid argument");
   if (count == 0) return LBM_OK;

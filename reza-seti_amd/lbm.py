@@ -127,6 +127,7 @@ SIGNATURES = [
     ("lbm_get_wss", [_P, _P, _P, _P], C.c_int),
     ("lbm_get_osi_tawss", [_P, _P, _P], C.c_int),
     ("lbm_get_macroscopic", [_P, _P, _P], C.c_int),
+    ("lbm_get_post_collision_moments", [_P, _P, C.c_int64, _P, _P], C.c_int),
     ("lbm_reduce", [_P, _P], C.c_int),
     ("lbm_zouhe", [C.c_int32, C.c_int64, _P, _P, _P], C.c_int),
     ("lbm_reduce_noalloc", [_P, _P], C.c_int),
@@ -500,6 +501,14 @@ class Solver:
         rho = np.zeros(self.n)
         u = np.zeros((self.n, 3))
         _check(lib().lbm_get_macroscopic(self._h, _ptr(rho), _ptr(u)))
+        return rho, u
+
+    def post_collision_moments(self, nodes):
+        """(rho, u) of the post-collision state at the given nodes."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        rho = np.zeros(len(nodes))
+        u = np.zeros((len(nodes), 3))
+        _check(lib().lbm_get_post_collision_moments(self._h, _ptr(nodes), len(nodes), _ptr(rho), _ptr(u)))
         return rho, u
 
     def reduce(self):

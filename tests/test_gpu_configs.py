@@ -1,0 +1,169 @@
+"""The benchmark's adjoint and large-share workloads at their real sizes.
+
+* Adjoint on the ~2.85M-node Y-bifurcation (16 Zernike inlet coefficients
+  times a pulsatile factor, a seeded Brinkman porosity field, the stress term
+  over a 20-lu daughter segment plus the pressure drop), 2400 steps with a
+  checkpoint every 200 (the flow reaches the segment, ~900 lu downstream of
+  the inlet, after ~1600 steps): the FP64 adjoint against central differences of the
+  FP64 forward (h = 1e-4 relative) along a random inlet direction and a
+  random porosity direction, to 1e-3; the FP32 adjoint against the FP64 one,
+  to 1e-3 in relative L2.
+* The ~300M-node tortuous-vessel share (its state exceeds 4 GiB, so the step
+  takes 64-bit offsets), 100 steps in FP32 and FP64. The vessel is closed
+  (its end nodes bounce back, LBM_INLET_NONE / LBM_OUTLET_NONE) and driven by
+  a body force along z, so its mass is conserved exactly and the FP64 mass
+  sum may drift by rounding only (1e-10 relative); FP32 storage against FP64
+  in wall shear stress (all near-wall nodes) and velocity (a seeded sample of
+  2M nodes), to 1e-4 in relative L2.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import lbm
+import workloads
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def bifurcation(tmp_path_factory):
+    geom, _, info = bench.build_geometry(str(tmp_path_factory.mktemp("bif")))
+    return geom, info
+
+
+def test_adjoint_on_the_bifurcation(bifurcation):
+    geom, info = bifurcation
+    T, K = 2400, 200
+    case = bench.adjoint_case(geom, info)
+    wave = case["wave"][:T]
+
+    def coef(th):
+        return wave[:, None] * th[None, :]
+
+    def J(prec, theta=None, alpha=None, **obj):
+        s = bench.adjoint_solver(geom, case, prec)
+        if alpha is not None:
+            s.set_porosity(alpha)
+        j = s.adjoint_forward(T, coef(case["theta"] if theta is None else theta), checkpoint_every=K, **obj)
+        return s, j
+
+    # the stress term weighted to the size of the pressure term, as the bench does
+    _, jw = J(lbm.PREC_F64, wss_weight=1.0, dp_weight=0.0, wss_mask=case["mask"])
+    _, jp = J(lbm.PREC_F64, wss_weight=0.0, dp_weight=1.0)
+    assert jw > 1e-12 * abs(jp)  # the stress term is live
+    obj = dict(wss_weight=abs(jp / jw), dp_weight=1.0, wss_mask=case["mask"])
+    print(f"{geom.n} nodes, {len(case['theta'])} inlet modes, J_wss {jw:.6e}, J_dp {jp:.6e}", flush=True)
+    s64, J64 = J(lbm.PREC_F64, **obj)
+    gc64, ga64 = s64.adjoint_backward(0)
+    s64.close()
+    gt64 = (wave[:, None] * gc64).sum(0)
+    # inlet direction
+    v = np.random.default_rng(7).standard_normal(len(case["theta"]))
+    v /= np.linalg.norm(v)
+    h = 1e-4 * np.linalg.norm(case["theta"])
+    fd = (J(lbm.PREC_F64, theta=case["theta"] + h * v, **obj)[1] -
+          J(lbm.PREC_F64, theta=case["theta"] - h * v, **obj)[1]) / (2 * h)
+    ad = float(gt64 @ v)
+    print(f"inlet direction: adjoint {ad:.9e}  FD {fd:.9e}", flush=True)
+    assert abs(ad - fd) <= 1e-3 * abs(fd)
+    # porosity direction (FP32 field: the perturbation as represented)
+    beta = np.random.default_rng(9).uniform(-1.0, 1.0, geom.n)  # 1% of the field, so it stays >= 0
+    a0 = case["alpha"].astype(np.float64)
+    ap = (a0 * (1.0 + 1e-2 * beta)).astype(np.float32)
+    am = (a0 * (1.0 - 1e-2 * beta)).astype(np.float32)
+    dv = (ap.astype(np.float64) - am.astype(np.float64)) / 2
+    fdp = J(lbm.PREC_F64, alpha=ap, **obj)[1] - J(lbm.PREC_F64, alpha=am, **obj)[1]
+    adp = 2 * float(ga64 @ dv)
+    print(f"porosity direction: adjoint {adp:.9e}  FD {fdp:.9e}", flush=True)
+    assert abs(adp - fdp) <= 1e-3 * abs(fdp)
+    # FP32 adjoint against FP64
+    s32, J32 = J(lbm.PREC_F32, **obj)
+    gc32, ga32 = s32.adjoint_backward(0)
+    s32.close()
+    gt32 = (wave[:, None] * gc32).sum(0)
+    e_in, e_por = rel_l2(gt32, gt64), rel_l2(ga32, ga64)
+    print(f"FP32 vs FP64: J {abs(J32 - J64) / abs(J64):.2e}, inlet gradient {e_in:.2e}, porosity gradient "
+          f"{e_por:.2e}", flush=True)
+    assert e_in <= 1e-3 and e_por <= 1e-3
+
+
+def _large_share_child():
+    """The large-share comparison, run in a process of its own: torch (which
+    casts the vessel's labels on the GPU) initialises its device before the
+    library loads, and the ~30 GB of host tables go with the process."""
+    import json
+
+    import torch
+
+    free, _ = torch.cuda.mem_get_info()
+    if free < 160e9:
+        print(json.dumps({"skip": f"needs ~160 GB of free device memory, {free / 1e9:.0f} GB free"}))
+        return
+    fluid, origin, info = workloads.tortuous_vessel(scale=bench.LARGE_SCALE)
+    nz = fluid.shape[2]
+    inl = lbm.plane_block("z", float(origin[2]), (0.0, 0.0, -1.0))
+    out_b = lbm.plane_block("z", float(origin[2] + nz - 1), (0.0, 0.0, 1.0))
+    geom = lbm.Geometry.from_labels(fluid, origin, 1.0, inlets=[inl], outlets=[out_b], triangles=info["triangles"],
+                                    normals=2, nthreads=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    del fluid
+    print(f"tortuous vessel: {geom.n} nodes, {geom.info.n_nearwall} near-wall", file=sys.stderr, flush=True)
+    steps, g = 100, 2e-6
+    sample = np.sort(np.random.default_rng(11).choice(geom.n, 2_000_000, replace=False)).astype(np.int32)
+    res = {"nodes": geom.n}
+    w64 = u64 = None
+    for name, prec in (("f64", lbm.PREC_F64), ("f32", lbm.PREC_F32)):
+        s = lbm.Solver(geom, precision=prec, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NONE,
+                       outlet_mode=lbm.OUTLET_NONE, tau=1.0, dt=1e-5, parity_checks=0, wss_every=1,
+                       force=(0.0, 0.0, g))
+        inf = s.info()
+        res[f"{name}_state_bytes"] = inf.q * inf.n_pad * (8 if prec == lbm.PREC_F64 else 4)
+        m0 = s.reduce().mass
+        s.step(steps)
+        st = s.reduce()
+        w, _, _ = s.wss()
+        _, u = s.post_collision_moments(sample)
+        s.close()
+        res[f"{name}_mass"] = [m0, st.mass]
+        if name == "f64":
+            w64, u64 = w, u
+            res["f64_max_u"], res["f64_max_wss"] = float(np.abs(u).max()), float(np.abs(w).max())
+        else:
+            res["wss_rel_l2"], res["velocity_rel_l2"] = rel_l2(w, w64), rel_l2(u, u64)
+        print(f"{name}: mass {m0!r} -> {st.mass!r}, wss_l2 {st.wss_l2:.6e}", file=sys.stderr, flush=True)
+    print(json.dumps(res))
+
+
+def test_large_share_fp32_against_fp64():
+    import json
+    import subprocess
+
+    env = dict(os.environ)
+    paths = [os.path.join(ROOT, "reza-seti_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests"), ROOT]
+    env["PYTHONPATH"] = os.pathsep.join(paths + ([env["PYTHONPATH"]] if env.get("PYTHONPATH") else []))
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--large-share-child"], stdout=subprocess.PIPE,
+                       text=True, timeout=900, env=env)
+    assert r.returncode == 0
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    if "skip" in res:
+        pytest.skip(res["skip"])
+    print(res)
+    assert res["nodes"] > 290_000_000
+    assert res["f32_state_bytes"] >= 2 ** 32 and res["f64_state_bytes"] >= 2 ** 32  # 64-bit offsets
+    m0, m1 = res["f64_mass"]
+    assert abs(m1 - m0) <= 1e-10 * m0
+    assert res["f64_max_u"] > 0 and res["f64_max_wss"] > 0
+    assert res["wss_rel_l2"] <= 1e-4 and res["velocity_rel_l2"] <= 1e-4
+
+
+if __name__ == "__main__" and sys.argv[1:] == ["--large-share-child"]:
+    _large_share_child()

@@ -22,6 +22,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../lattice.h"
 #include "collision.h"
 #include "kernels.h"
@@ -375,8 +377,38 @@ __global__ __launch_bounds__(BLOCK) void finish_kernel(const double* partials, i
   }
 }
 
+// The step as two launches: the special slots (boundary rules, stress,
+// previous values; many registers) and the bulk slots (pull, collide, store;
+// few registers, so many more wavefronts in flight). Together they write what
+// one lbm_kernel launch writes.
+template <class L, class S, int MODE, bool FORCE>
+__global__ __launch_bounds__(BLOCK) void lbm_special_kernel(StepArgs a) {
+  const int32_t p = blockIdx.x * BLOCK + threadIdx.x;
+  if (p < a.n_special_pad) Kern<L, S>::template special_node<MODE, FORCE>(a, p);
+}
+
+template <class L, class S, int MODE, bool FORCE>
+__global__ __launch_bounds__(BLOCK) void lbm_bulk_kernel(StepArgs a) {
+  const int32_t b = blockIdx.x;
+  const int x = b & 7;
+  const int32_t chunk = a.xcd_remap ? a.xcdb_offset[x] + (b - x) / 8 : b;
+  const int32_t p = a.n_special_pad + chunk * BLOCK + threadIdx.x;
+  if (p < a.bulk_end) Kern<L, S>::template bulk_node<MODE, FORCE>(a, p);
+}
+
+#ifndef LBM_REF_SPLIT
+#define LBM_REF_SPLIT 1
+#endif
+
 template <class L, class S, int MODE, bool FORCE>
 hipError_t launch_one(const StepArgs& a, hipStream_t st) {
+  if constexpr (LBM_REF_SPLIT && MODE == KMODE_STEP && std::is_same<S, double>::value) {
+    if (a.blocks_special) hipLaunchKernelGGL((lbm_special_kernel<L, S, MODE, FORCE>), dim3(a.blocks_special), dim3(BLOCK), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !a.blocks_bulk) return e;
+    hipLaunchKernelGGL((lbm_bulk_kernel<L, S, MODE, FORCE>), dim3(a.blocks_bulk), dim3(BLOCK), 0, st, a);
+    return hipGetLastError();
+  }
   const dim3 grid(a.blocks_special + a.blocks_bulk);
   if (grid.x == 0) return hipSuccess;
   hipLaunchKernelGGL((lbm_kernel<L, S, MODE, FORCE>), grid, dim3(BLOCK), 0, st, a);

@@ -813,16 +813,25 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
     return LBM_OK;
   }
 
-  // Slot layout: special nodes (wall, inlet, outlet, near-wall) first, then
-  // plain fluid, each in reference order.
+  // Slot layout: special nodes first, grouped by the rule they run (inlet and
+  // outlet nodes, wall nodes, near-wall plain fluid nodes), each group in
+  // reference order and padded to whole wavefronts, so that a wavefront runs
+  // one rule instead of all of them in turn; then plain fluid in reference
+  // order. Slot order is internal: every result is reported per node.
   const int64_t N = t.N;
   std::vector<uint8_t> nearwall(N, 0);
   for (int32_t n : t.nearwall) nearwall[n] = 1;
   s->slot_of.assign(N, -1);
   int64_t ns = 0;
-  for (int64_t n = 0; n < N; ++n)
-    if (t.kind[n] != KIND_FLUID || nearwall[n]) s->slot_of[n] = static_cast<int32_t>(ns++);
-  s->n_special = ns;
+  auto group = [&](auto member) {
+    for (int64_t n = 0; n < N; ++n)
+      if (member(n)) s->slot_of[n] = static_cast<int32_t>(ns++);
+    ns = round_up(ns, 64);
+  };
+  group([&](int64_t n) { return t.kind[n] == KIND_INLET || t.kind[n] == KIND_OUTLET; });
+  group([&](int64_t n) { return t.kind[n] == KIND_WALL; });
+  group([&](int64_t n) { return t.kind[n] == KIND_FLUID && nearwall[n]; });
+  s->n_special = ns;  // padding slots inside the range carry SFLAG_PAD
   s->n_special_pad = round_up(ns, BLOCK);
   int64_t nb = 0;
   for (int64_t n = 0; n < N; ++n)
@@ -859,6 +868,7 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
   std::vector<double> nrm(3 * sp, 0.0);
   for (int64_t slot = 0; slot < s->n_special; ++slot) {
     const int32_t n = s->node_of[slot];
+    if (n < 0) continue;  // padding between groups
     sflags[slot] = uint32_t(t.kind[n]) | (nearwall[n] ? SFLAG_NEARWALL : 0u);
     if (t.bc_nbr[n] >= 0) bcn[slot] = s->slot_of[t.bc_nbr[n]];
   }
@@ -919,6 +929,11 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
       a.xcd_first[x] = first;
       a.xcd_offset[x] = off;
       off += cnt;
+    }
+    off = 0;
+    for (int x = 0; x < 8; ++x) {
+      a.xcdb_offset[x] = off;
+      off += x < Bb ? (Bb - 1 - x) / 8 + 1 : 0;
     }
   }
   a.sflags = s->sflags;
@@ -1828,7 +1843,7 @@ int lbm_get_solver_info(const lbm_solver* s, lbm_solver_info* o) {
   if (!s || !o) return fail(LBM_ERR_ARG, "null argument");
   std::memset(o, 0, sizeof(*o));
   o->n_nodes = s->N;
-  o->n_special = s->n_special;
+  o->n_special = s->N - s->n_bulk;
   o->n_bulk = s->n_bulk;
   if (s->prm.wall_mode == LBM_WALL_HALFWAY_BB) {  // column segments: explicit rows vs segment-decoded slots
     o->n_special = s->n_irr;

@@ -302,7 +302,8 @@ def run_large_share(steps, warmup=20):
     inl = lbm.plane_block("z", float(origin[2]), (0.0, 0.0, -1.0))
     out_b = lbm.plane_block("z", float(origin[2] + nz - 1), (0.0, 0.0, 1.0))
     geom = lbm.Geometry.from_labels(fluid, origin, 1.0, inlets=[inl], outlets=[out_b], triangles=info["triangles"],
-                                    normals=2, nthreads=int(os.environ.get("OMP_NUM_THREADS", "16")))
+                                    normals=2, nthreads=int(os.environ.get("OMP_NUM_THREADS", "16")),
+                                    classify_device=True)
     grid = tuple(int(v) for v in fluid.shape)
     del fluid
     t2 = time.perf_counter()

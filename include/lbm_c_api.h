@@ -116,8 +116,10 @@ typedef struct {
   int32_t nthreads;        /* host threads, 0 = all */
   int32_t verify_stride;   /* >0: re-check every n-th column against all triangles */
   int32_t compute_normals; /* near-wall normals (Subdomain.cpp:499-572): 0 none, 1 host, 2 GPU */
-  int32_t device;          /* HIP device for compute_normals = 2 and label_device = 1 */
+  int32_t device;          /* HIP device for compute_normals = 2, label_device = 1 and classify_device = 1 */
   int32_t label_device;    /* ray labels (ComplexGeometry.cpp:440-500): 0 host threads, 1 GPU (same labels) */
+  int32_t classify_device; /* boundary points, node kinds, order and push table (ComplexGeometry.cpp:517-662,
+                              Subdomain.cpp:364-471): 0 host threads, 1 GPU (same tables) */
 } lbm_geometry_options;
 
 void lbm_geometry_options_default(lbm_geometry_options* o);

@@ -56,6 +56,7 @@ static BuildOptions to_opts(const lbm_geometry_options* o) {
   b.compute_normals = o->compute_normals;
   b.device = o->device;
   b.label_device = o->label_device;
+  b.classify_device = o->classify_device;
   return b;
 }
 

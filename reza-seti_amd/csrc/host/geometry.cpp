@@ -14,6 +14,7 @@
 #include <unordered_set>
 
 #include "../lattice.h"
+#include "classify.h"
 #include "normals.h"
 #include "raytest.h"
 #include "voxel.h"
@@ -316,6 +317,64 @@ bool check_block(const PlaneSpec& s, const char* what, std::string* err) {
 
 }  // namespace
 
+namespace {
+
+// Near-wall set and its normals from finished node tables (the last steps of
+// build_from_labels, shared by the host and the device classification).
+bool nearwall_and_normals(const GridSpec& g, const std::vector<Triangle>& tris, const BuildOptions& o, Tables& t,
+                          std::string* err) {
+  const int64_t N = t.N;
+  const int Q = t.Q;
+  // Near-wall set (SubDomain::IdentifyNextToWallLattices, Subdomain.cpp:475-493):
+  // plain-fluid neighbours of wall nodes. The reference keeps them in an
+  // unordered_set (output-only order); here they are sorted by node index.
+  {
+    std::vector<uint8_t> mark(N, 0);
+    for (int32_t w : t.wall_nodes)
+      for (int q = 0; q < Q; ++q) {
+        const int32_t nbr = t.nb[static_cast<size_t>(q) * N + w];
+        if (nbr >= 0 && t.kind[nbr] == KIND_FLUID) mark[nbr] = 1;
+      }
+    t.nearwall.clear();
+    for (int64_t n = 0; n < N; ++n)
+      if (mark[n]) t.nearwall.push_back(static_cast<int32_t>(n));
+  }
+  t.normals.assign(3 * t.nearwall.size(), 0.0);
+  if (o.compute_normals) {
+    std::vector<TriTerms> tt;
+    tt.reserve(tris.size());
+    for (const Triangle& T : tris) tt.push_back(tri_terms(T));
+    std::vector<double> pts(3 * t.nearwall.size());
+    for (size_t a = 0; a < t.nearwall.size(); ++a) {
+      const int32_t n = t.nearwall[a];
+      pts[3 * a] = g.px(t.ijk[3 * n]);
+      pts[3 * a + 1] = g.py(t.ijk[3 * n + 1]);
+      pts[3 * a + 2] = g.pz(t.ijk[3 * n + 2]);
+    }
+    if (o.compute_normals == 2) {
+      // Same sum evaluated on the GPU (bit identical; see normals.hip).
+      std::vector<double> out;
+      if (!device_boundary_normals(pts, tt, out, o.device, err)) return false;
+      t.normals.swap(out);
+    } else {
+      parallel_for(static_cast<int64_t>(t.nearwall.size()), o.nthreads, [&](int64_t a) {
+        const auto nv = normal_from_terms(pts[3 * a], pts[3 * a + 1], pts[3 * a + 2], tt);
+        t.normals[3 * a] = nv[0];
+        t.normals[3 * a + 1] = nv[1];
+        t.normals[3 * a + 2] = nv[2];
+      });
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+bool classify_on_device(const GridSpec& g, const std::vector<uint8_t>& fluid,
+                               const std::vector<PlaneSpec>& inlets, const std::vector<PlaneSpec>& outlets,
+                               const std::vector<Triangle>& tris, const BuildOptions& o, Tables& t,
+                               std::string* err);
+
 bool build_from_labels(const GridSpec& g, const std::vector<uint8_t>& fluid,
                        const std::vector<PlaneSpec>& inlets, const std::vector<PlaneSpec>& outlets,
                        const std::vector<Triangle>& tris, const BuildOptions& o, Tables& t,
@@ -334,6 +393,7 @@ bool build_from_labels(const GridSpec& g, const std::vector<uint8_t>& fluid,
     if (!check_block(s, "inlet", err)) return false;
   for (const auto& s : outlets)
     if (!check_block(s, "outlet", err)) return false;
+  if (o.classify_device) return classify_on_device(g, fluid, inlets, outlets, tris, o, t, err);
 
   const LatticeView L = lattice_view(o.Q);
   const int NX = g.NX, NY = g.NY, NZ = g.NZ;
@@ -482,47 +542,82 @@ bool build_from_labels(const GridSpec& g, const std::vector<uint8_t>& fluid,
   if (t.has_inlet_normal)
     for (int32_t n : t.inlet_nodes) t.bc_nbr[n] = behind(n, t.inlet_normal);
 
-  // Near-wall set (SubDomain::IdentifyNextToWallLattices, Subdomain.cpp:475-493):
-  // plain-fluid neighbours of wall nodes. The reference keeps them in an
-  // unordered_set (output-only order); here they are sorted by node index.
-  {
-    std::vector<uint8_t> mark(N, 0);
-    for (int32_t w : t.wall_nodes)
-      for (int q = 0; q < Q; ++q) {
-        const int32_t nbr = t.nb[static_cast<size_t>(q) * N + w];
-        if (nbr >= 0 && t.kind[nbr] == KIND_FLUID) mark[nbr] = 1;
+  return nearwall_and_normals(g, tris, o, t, err);
+}
+
+// build_from_labels with the boundary points, node kinds, node order and
+// push table computed on the GPU (device_classify); the creation-order lists,
+// the nodes behind inlets/outlets and the near-wall set follow from those
+// tables here, with the same rules.
+bool classify_on_device(const GridSpec& g, const std::vector<uint8_t>& fluid,
+                        const std::vector<PlaneSpec>& inlets, const std::vector<PlaneSpec>& outlets,
+                        const std::vector<Triangle>& tris, const BuildOptions& o, Tables& t, std::string* err) {
+  DeviceTables d;
+  if (!device_classify(g, fluid, inlets, outlets, o.Q, o.periodic, o.device, d, err)) return false;
+  const LatticeView L = lattice_view(o.Q);
+  const int Q = L.Q;
+  const int64_t N = d.N;
+  t = Tables();
+  t.grid = g;
+  t.Q = o.Q;
+  for (int a = 0; a < 3; ++a) t.periodic[a] = o.periodic[a];
+  t.fluid = fluid;
+  t.N = N;
+  t.ijk = std::move(d.ijk);
+  t.kind = std::move(d.kind);
+  t.outlet_id = std::move(d.outlet_id);
+  t.nb = std::move(d.nb);
+  t.n_boundary_points = d.n_boundary_points;
+  t.outlet_nodes.assign(outlets.size(), {});
+  for (const auto& s : outlets) {
+    std::array<double, 3> n{0, 0, 0};
+    for (size_t a = 0; a < 3 && a < s.normal.size(); ++a) n[a] = s.normal[a];
+    t.outlet_normals.push_back(n);
+  }
+  if (!inlets.empty()) {
+    const auto& nrm = inlets.back().normal;  // only the last block's normal survives (ComplexGeometry.cpp:605)
+    if (nrm.size() < 3) {
+      if (err) *err = "inlet normal must have 3 components";
+      return false;
+    }
+    t.inlet_normal = {nrm[0], nrm[1], nrm[2]};
+    t.has_inlet_normal = true;
+  }
+  for (int64_t n = 0; n < N; ++n) {
+    if (t.kind[n] == KIND_INLET) t.inlet_nodes.push_back(static_cast<int32_t>(n));
+    else if (t.kind[n] == KIND_WALL) t.wall_nodes.push_back(static_cast<int32_t>(n));
+    else if (t.kind[n] == KIND_OUTLET) t.outlet_nodes[t.outlet_id[n]].push_back(static_cast<int32_t>(n));
+  }
+  // The node behind an inlet/outlet node along its block normal (else-if
+  // order +x,-x,+y,-y,+z,-z), from the push table: the neighbour in that
+  // axis direction, or none outside a non-periodic box.
+  t.bc_nbr.assign(N, -1);
+  const int dims[3] = {g.NX, g.NY, g.NZ};
+  auto behind = [&](int32_t n, const std::array<double, 3>& nv) -> int32_t {
+    int dv[3] = {0, 0, 0};
+    if (nv[0] > 0) dv[0] = -1;
+    else if (nv[0] < 0) dv[0] = 1;
+    else if (nv[1] > 0) dv[1] = -1;
+    else if (nv[1] < 0) dv[1] = 1;
+    else if (nv[2] > 0) dv[2] = -1;
+    else if (nv[2] < 0) dv[2] = 1;
+    else return -1;
+    for (int a = 0; a < 3; ++a) {
+      const int v = t.ijk[3 * int64_t(n) + a] + dv[a];
+      if ((v < 0 || v >= dims[a]) && !o.periodic[a]) {
+        t.upstream_out_of_box++;
+        return -1;
       }
-    t.nearwall.clear();
-    for (int64_t n = 0; n < N; ++n)
-      if (mark[n]) t.nearwall.push_back(static_cast<int32_t>(n));
-  }
-  t.normals.assign(3 * t.nearwall.size(), 0.0);
-  if (o.compute_normals) {
-    std::vector<TriTerms> tt;
-    tt.reserve(tris.size());
-    for (const Triangle& T : tris) tt.push_back(tri_terms(T));
-    std::vector<double> pts(3 * t.nearwall.size());
-    for (size_t a = 0; a < t.nearwall.size(); ++a) {
-      const int32_t n = t.nearwall[a];
-      pts[3 * a] = g.px(t.ijk[3 * n]);
-      pts[3 * a + 1] = g.py(t.ijk[3 * n + 1]);
-      pts[3 * a + 2] = g.pz(t.ijk[3 * n + 2]);
     }
-    if (o.compute_normals == 2) {
-      // Same sum evaluated on the GPU (bit identical; see normals.hip).
-      std::vector<double> out;
-      if (!device_boundary_normals(pts, tt, out, o.device, err)) return false;
-      t.normals.swap(out);
-    } else {
-      parallel_for(static_cast<int64_t>(t.nearwall.size()), o.nthreads, [&](int64_t a) {
-        const auto nv = normal_from_terms(pts[3 * a], pts[3 * a + 1], pts[3 * a + 2], tt);
-        t.normals[3 * a] = nv[0];
-        t.normals[3 * a + 1] = nv[1];
-        t.normals[3 * a + 2] = nv[2];
-      });
-    }
-  }
-  return true;
+    for (int q = 0; q < Q; ++q)
+      if (L.cx[q] == dv[0] && L.cy[q] == dv[1] && L.cz[q] == dv[2]) return t.nb[static_cast<size_t>(q) * N + n];
+    return -1;
+  };
+  for (size_t s = 0; s < t.outlet_nodes.size(); ++s)
+    for (int32_t n : t.outlet_nodes[s]) t.bc_nbr[n] = behind(n, t.outlet_normals[s]);
+  if (t.has_inlet_normal)
+    for (int32_t n : t.inlet_nodes) t.bc_nbr[n] = behind(n, t.inlet_normal);
+  return nearwall_and_normals(g, tris, o, t, err);
 }
 
 bool build_from_tables(const TableInput& in, Tables& t, std::string* err) {

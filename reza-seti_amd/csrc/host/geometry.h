@@ -65,6 +65,7 @@ struct BuildOptions {
   int compute_normals = 1;       // 0 none, 1 host threads, 2 GPU (same sum)
   int device = 0;
   int label_device = 0;          // ray labels: 0 host threads, 1 GPU (same labels)
+  int classify_device = 0;       // boundary points, kinds, order, push table: 0 host, 1 GPU (same tables)
 };
 
 struct Tables {

@@ -49,7 +49,7 @@ class Config(C.Structure):
 class GeometryOptions(C.Structure):
     _fields_ = [("q", C.c_int32), ("periodic", C.c_int32 * 3), ("nthreads", C.c_int32),
                 ("verify_stride", C.c_int32), ("compute_normals", C.c_int32), ("device", C.c_int32),
-                ("label_device", C.c_int32)]
+                ("label_device", C.c_int32), ("classify_device", C.c_int32)]
 
 
 class GeometryInfo(C.Structure):
@@ -250,10 +250,11 @@ class Geometry:
         _check(lib().lbm_geometry_get_info(self._h, C.byref(self.info)))
 
     @staticmethod
-    def _opts(q, periodic, nthreads, verify_stride, normals, label_device=False):
+    def _opts(q, periodic, nthreads, verify_stride, normals, label_device=False, classify_device=False):
         o = GeometryOptions()
         lib().lbm_geometry_options_default(C.byref(o))
         o.label_device = int(label_device)
+        o.classify_device = int(classify_device)
         o.q = q
         for a in range(3):
             o.periodic[a] = int(periodic[a])
@@ -264,19 +265,20 @@ class Geometry:
 
     @classmethod
     def from_config(cls, path, q=19, periodic=(0, 0, 0), nthreads=0, verify_stride=0, normals=True,
-                    label_device=False):
-        """label_device=True casts the voxeliser's rays on the GPU (same labels)."""
-        o = cls._opts(q, periodic, nthreads, verify_stride, normals, label_device)
+                    label_device=False, classify_device=False):
+        """label_device=True casts the voxeliser's rays on the GPU (same labels);
+        classify_device=True builds the node tables there (same tables)."""
+        o = cls._opts(q, periodic, nthreads, verify_stride, normals, label_device, classify_device)
         h = C.c_void_p()
         _check(lib().lbm_geometry_from_config(str(path).encode(), C.byref(o), C.byref(h)))
         return cls(h.value)
 
     @classmethod
     def from_labels(cls, fluid, origin, spacing, inlets=(), outlets=(), triangles=None, q=19,
-                    periodic=(0, 0, 0), nthreads=0, normals=True):
+                    periodic=(0, 0, 0), nthreads=0, normals=True, classify_device=False):
         fluid = np.ascontiguousarray(fluid, dtype=np.uint8)
         nx, ny, nz = fluid.shape
-        o = cls._opts(q, periodic, nthreads, 0, normals)
+        o = cls._opts(q, periodic, nthreads, 0, normals, classify_device=classify_device)
         org = np.ascontiguousarray(origin, dtype=np.float64)
         ins = (PlaneBlock * max(1, len(inlets)))(*inlets)
         outs = (PlaneBlock * max(1, len(outlets)))(*outlets)

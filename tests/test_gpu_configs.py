@@ -9,7 +9,9 @@
   random porosity direction, to 1e-3; the FP32 adjoint against the FP64 one,
   to 1e-3 in relative L2.
 * The ~300M-node tortuous-vessel share (its state exceeds 4 GiB, so the step
-  takes 64-bit offsets), 100 steps in FP32 and FP64. The vessel is closed
+  takes 64-bit offsets): its tables classified on the GPU (classify_device)
+  equal the host chain's (128-bit digests of every table), then 100 steps in
+  FP32 and FP64. The vessel is closed
   (its end nodes bounce back, LBM_INLET_NONE / LBM_OUTLET_NONE) and driven by
   a body force along z, so its mass is conserved exactly and the FP64 mass
   sum may drift by rounding only (1e-10 relative); FP32 storage against FP64
@@ -41,6 +43,7 @@ def bifurcation(tmp_path_factory):
     return geom, info
 
 
+@pytest.mark.timeout(900)
 def test_adjoint_on_the_bifurcation(bifurcation):
     geom, info = bifurcation
     T, K = 2400, 200
@@ -101,9 +104,11 @@ def _large_share_child():
     """The large-share comparison, run in a process of its own: torch (which
     casts the vessel's labels on the GPU) initialises its device before the
     library loads, and the ~30 GB of host tables go with the process."""
+    import gc
     import json
 
     import torch
+    import xxhash
 
     free, _ = torch.cuda.mem_get_info()
     if free < 160e9:
@@ -113,13 +118,35 @@ def _large_share_child():
     nz = fluid.shape[2]
     inl = lbm.plane_block("z", float(origin[2]), (0.0, 0.0, -1.0))
     out_b = lbm.plane_block("z", float(origin[2] + nz - 1), (0.0, 0.0, 1.0))
-    geom = lbm.Geometry.from_labels(fluid, origin, 1.0, inlets=[inl], outlets=[out_b], triangles=info["triangles"],
-                                    normals=2, nthreads=int(os.environ.get("OMP_NUM_THREADS", "16")))
+
+    def build(on_device):
+        return lbm.Geometry.from_labels(fluid, origin, 1.0, inlets=[inl], outlets=[out_b],
+                                        triangles=info["triangles"], normals=2,
+                                        nthreads=int(os.environ.get("OMP_NUM_THREADS", "16")),
+                                        classify_device=on_device)
+
+    def digest(g):
+        out = [tuple(getattr(g.info, k) for k in ("n_nodes", "n_wall", "n_inlet", "n_outlet", "n_nearwall",
+                                                   "n_boundary_points", "upstream_out_of_box"))]
+        for get in (g.nodes, lambda: (g.neighbours(),), lambda: (g.bc_neighbour(),), g.nearwall):
+            h = xxhash.xxh3_128()
+            for a in get():
+                h.update(np.ascontiguousarray(a).view(np.uint8).reshape(-1))
+            out.append(h.hexdigest())
+            gc.collect()
+        return out
+
+    host = build(False)
+    host_digest = digest(host)
+    host.close()
+    del host
+    gc.collect()
+    geom = build(True)
     del fluid
     print(f"tortuous vessel: {geom.n} nodes, {geom.info.n_nearwall} near-wall", file=sys.stderr, flush=True)
+    res = {"nodes": geom.n, "device_tables_equal_host": digest(geom) == host_digest}
     steps, g = 100, 2e-6
     sample = np.sort(np.random.default_rng(11).choice(geom.n, 2_000_000, replace=False)).astype(np.int32)
-    res = {"nodes": geom.n}
     w64 = u64 = None
     for name, prec in (("f64", lbm.PREC_F64), ("f32", lbm.PREC_F32)):
         s = lbm.Solver(geom, precision=prec, wall_mode=lbm.WALL_HALFWAY_BB, inlet_mode=lbm.INLET_NONE,
@@ -143,6 +170,7 @@ def _large_share_child():
     print(json.dumps(res))
 
 
+@pytest.mark.timeout(900)
 def test_large_share_fp32_against_fp64():
     import json
     import subprocess
@@ -158,6 +186,7 @@ def test_large_share_fp32_against_fp64():
         pytest.skip(res["skip"])
     print(res)
     assert res["nodes"] > 290_000_000
+    assert res["device_tables_equal_host"]
     assert res["f32_state_bytes"] >= 2 ** 32 and res["f64_state_bytes"] >= 2 ** 32  # 64-bit offsets
     m0, m1 = res["f64_mass"]
     assert abs(m1 - m0) <= 1e-10 * m0

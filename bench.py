@@ -287,31 +287,44 @@ def run_stenosis(steps, warmup=20):
 LARGE_SCALE, LARGE_WO, LARGE_PERIOD, LARGE_WSS_EVERY = 4.3, 5.0, 8000, 100
 
 
+def large_share_geometry(host_chain=False):
+    """The large share's geometry, built the reference's way from a surface:
+    the tortuous vessel's STL (workloads.tortuous_vessel_stl, marching
+    tetrahedra at 8 lu) through the set-up chain with ray labels,
+    classification, order, push table and near-wall normals on the GPU
+    (host_chain=True: labels and tables on 16 host threads instead). Returns
+    (geometry, vessel info, set-up seconds)."""
+    import tempfile
+
+    t0 = time.perf_counter()
+    with tempfile.TemporaryDirectory() as d:
+        cfg, info = workloads.tortuous_vessel_stl(d, scale=LARGE_SCALE, mesh_h=8.0)
+        t1 = time.perf_counter()
+        print(f"# large share: STL of {info['n_triangles']} triangles in {t1 - t0:.1f}s", file=sys.stderr, flush=True)
+        on_gpu = not host_chain
+        geom = lbm.Geometry.from_config(cfg, normals=2, label_device=on_gpu, classify_device=on_gpu,
+                                        nthreads=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    t2 = time.perf_counter()
+    print(f"# large share: {geom.n} nodes, geometry {t2 - t1:.1f}s", file=sys.stderr, flush=True)
+    return geom, info, {"stl": round(t1 - t0, 1), "geometry": round(t2 - t1, 1)}
+
+
 def run_large_share(steps, warmup=20):
     """Large single-GPU share: the tortuous vessel (Catmull-Rom centreline
     through a seeded 12-point random walk, radius 20-40 and a saccular bulge
-    of radius 60, all lengths x4.3: ~300M nodes), D3Q19 FP32-shifted,
-    Womersley inlet (Wo 5, period 8000 steps, U_mean 0.05), NEE zero-pressure
-    outlet, bounce-back walls, wall shear stress every 100 steps. Returns the
-    step rate and its roofline fraction, with the set-up times."""
-    t0 = time.perf_counter()
-    fluid, origin, info = workloads.tortuous_vessel(scale=LARGE_SCALE)
-    t1 = time.perf_counter()
-    print(f"# large share: labels {fluid.shape} in {t1 - t0:.1f}s", file=sys.stderr, flush=True)
-    nz = fluid.shape[2]
-    inl = lbm.plane_block("z", float(origin[2]), (0.0, 0.0, -1.0))
-    out_b = lbm.plane_block("z", float(origin[2] + nz - 1), (0.0, 0.0, 1.0))
-    geom = lbm.Geometry.from_labels(fluid, origin, 1.0, inlets=[inl], outlets=[out_b], triangles=info["triangles"],
-                                    normals=2, nthreads=int(os.environ.get("OMP_NUM_THREADS", "16")),
-                                    classify_device=True)
-    grid = tuple(int(v) for v in fluid.shape)
-    del fluid
+    of radius 60, all lengths x4.3: ~300M nodes) built from its STL through
+    the GPU set-up chain, D3Q19 FP32-shifted, Womersley inlet (Wo 5, period
+    8000 steps, U_mean 0.05), NEE zero-pressure outlet, bounce-back walls,
+    wall shear stress every 100 steps. Returns the step rate and its roofline
+    fraction, with the set-up times."""
+    geom, info, setup = large_share_geometry()
     t2 = time.perf_counter()
-    print(f"# large share: {geom.n} nodes, geometry {t2 - t1:.1f}s", file=sys.stderr, flush=True)
+    grid = tuple(int(v) for v in geom.info.grid)
     ijk, kind, _ = geom.nodes()
     ii = np.where(kind == lbm.KIND_INLET)[0]
+    b, h = geom.info.bbox, geom.info.spacing
     c0, r0 = info["centreline"][0], float(info["radius"][0])
-    r = np.hypot(origin[0] + ijk[ii, 0] - c0[0], origin[1] + ijk[ii, 1] - c0[1]) / r0
+    r = np.hypot(b[0] + ijk[ii, 0] * h - c0[0], b[2] + ijk[ii, 1] * h - c0[1]) / r0
     del ijk, kind
     basis = womersley_basis(r, LARGE_WO)
     w = 2.0 * math.pi / LARGE_PERIOD
@@ -336,12 +349,13 @@ def run_large_share(steps, warmup=20):
     n = geom.n
     achieved = n * 152.0 / (ev_ms / 1e3 / steps) / 1e9
     res = {"workload": f"tortuous vessel x{LARGE_SCALE} (12-point walk, radius {20 * LARGE_SCALE:.0f}-{40 * LARGE_SCALE:.0f}, "
-                       f"bulge {60 * LARGE_SCALE:.0f}), D3Q19 FP32-shifted, Womersley Wo {LARGE_WO} period {LARGE_PERIOD}, "
+                       f"bulge {60 * LARGE_SCALE:.0f}) from its STL through GPU labels, classification and normals, "
+                       f"D3Q19 FP32-shifted, Womersley Wo {LARGE_WO} period {LARGE_PERIOD}, "
                        f"WSS every {LARGE_WSS_EVERY} steps",
            "nodes": n, "grid": grid, "triangles": info["n_triangles"], "steps": steps, "tau": round(tau, 4),
            "f32_mlups": round(n * steps / wall / 1e6, 1), "roofline_frac": round(achieved / PEAK_HBM_GBS, 4),
            "achieved_gbs": round(achieved, 1), "launch_us": round(ev_ms * 1e3 / steps, 2),
-           "setup_s": {"labels": round(t1 - t0, 1), "geometry": round(t2 - t1, 1), "solver": round(t3 - t2, 1)}}
+           "setup_s": dict(setup, solver=round(t3 - t2, 1))}
     s.close()
     return res
 

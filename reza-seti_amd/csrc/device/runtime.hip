@@ -10,6 +10,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cmath>
 #include <cstring>
@@ -20,6 +21,7 @@
 #include "../capi_internal.h"
 #include "../host/checkpoint.h"
 #include "../host/output.h"
+#include "../host/parallel.h"
 #include "../lattice.h"
 #include "adjoint.h"
 #include "columns.h"
@@ -436,10 +438,10 @@ int create_columns(lbm_solver* s, const Tables& t) {
   const int64_t npad = s->npad;
   s->slot_of.resize(N);
   s->node_of.assign(npad, -1);
-  for (int64_t n = 0; n < N; ++n) {
+  lbm::parallel_for(N, 0, [&](int64_t n) {
     s->slot_of[n] = static_cast<int32_t>(n);
     s->node_of[n] = static_cast<int32_t>(n);
-  }
+  });
   s->n_special = 0;
   s->n_special_pad = 0;
   s->n_bulk = N;
@@ -450,6 +452,28 @@ int create_columns(lbm_solver* s, const Tables& t) {
     return q == L.rest ? static_cast<int32_t>(n) : t.nb[static_cast<size_t>(L.opp[q]) * N + n];
   };
   const bool inl_b = p.inlet_mode != LBM_INLET_NONE, out_b = p.outlet_mode != LBM_OUTLET_NONE;
+  auto bc_node = [&](int64_t n) {
+    return (t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b);
+  };
+  // A node whose (i, j) differs from its predecessor's, or whose k does not
+  // follow it, starts a column run; every segment lies inside one run.
+  auto run_start = [&](int64_t n) {
+    if (n == 0) return true;
+    const int32_t* c = &t.ijk[3 * static_cast<size_t>(n)];
+    return c[0] != c[-3] || c[1] != c[-2] || c[2] != c[-1] + 1;
+  };
+  // Host ranges of whole runs, one per thread: the encodings below restart
+  // at every run, so the ranges are independent.
+  const int nth = lbm::resolve_threads(0);
+  std::vector<int64_t> bounds{0};
+  for (int r = 1; r < nth; ++r) {
+    int64_t b = std::max(bounds.back() + 1, N * r / nth);
+    while (b < N && !run_start(b)) ++b;
+    if (b >= N) break;
+    bounds.push_back(b);
+  }
+  bounds.push_back(N);
+  const int nranges = static_cast<int>(bounds.size()) - 1;
 
   // Column segments: runs of slots with the same (i, j), consecutive k and,
   // for every lateral offset o, one delta src + c_z - n that all existing
@@ -461,43 +485,58 @@ int create_columns(lbm_solver* s, const Tables& t) {
   // constrains nothing. Deltas no link establishes stay 0.
   std::vector<int32_t> seg_of(N), delta;
   {
-    int32_t cur[NCOL];
-    bool have[NCOL] = {};
-    for (int64_t n = 0; n < N; ++n) {
-      const int32_t* c = &t.ijk[3 * static_cast<size_t>(n)];
-      bool split = n == 0 || c[0] != c[-3] || c[1] != c[-2] || c[2] != c[-1] + 1;
-      bool self_ok = !((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b));
-      int32_t nd[NCOL];
-      bool nh[NCOL] = {};
-      for (int q = 0; q < Q && self_ok; ++q) {
-        const int32_t sq = srcq(q, n);
-        if (sq < 0) continue;
-        const int o = col_of(L.cx[q], L.cy[q]);
-        if (o < 0) {
-          if (int64_t(sq) != n - L.cz[q]) self_ok = false;
-          continue;
-        }
-        const int32_t dl = static_cast<int32_t>(int64_t(sq) + L.cz[q] - n);
-        if (nh[o] && nd[o] != dl) self_ok = false;
-        nd[o] = dl;
-        nh[o] = true;
-      }
-      if (self_ok && !split)
-        for (int o = 0; o < NCOL; ++o)
-          if (nh[o] && have[o] && cur[o] != nd[o]) split = true;
-      if (split) {
-        delta.insert(delta.end(), NCOL, 0);
-        std::fill(have, have + NCOL, false);
-      }
-      if (self_ok)
-        for (int o = 0; o < NCOL; ++o)
-          if (nh[o] && !have[o]) {
-            cur[o] = nd[o];
-            have[o] = true;
-            delta[delta.size() - NCOL + o] = nd[o];
+    std::vector<std::vector<int32_t>> part(nranges);
+    lbm::parallel_ranges(bounds, [&](int r, int64_t a, int64_t b) {
+      std::vector<int32_t>& dl_out = part[r];
+      int32_t cur[NCOL];
+      bool have[NCOL] = {};
+      for (int64_t n = a; n < b; ++n) {
+        bool split = n == a || run_start(n);
+        bool self_ok = !bc_node(n);
+        int32_t nd[NCOL];
+        bool nh[NCOL] = {};
+        for (int q = 0; q < Q && self_ok; ++q) {
+          const int32_t sq = srcq(q, n);
+          if (sq < 0) continue;
+          const int o = col_of(L.cx[q], L.cy[q]);
+          if (o < 0) {
+            if (int64_t(sq) != n - L.cz[q]) self_ok = false;
+            continue;
           }
-      seg_of[n] = static_cast<int32_t>(delta.size() / NCOL - 1);
+          const int32_t dl = static_cast<int32_t>(int64_t(sq) + L.cz[q] - n);
+          if (nh[o] && nd[o] != dl) self_ok = false;
+          nd[o] = dl;
+          nh[o] = true;
+        }
+        if (self_ok && !split)
+          for (int o = 0; o < NCOL; ++o)
+            if (nh[o] && have[o] && cur[o] != nd[o]) split = true;
+        if (split) {
+          dl_out.insert(dl_out.end(), NCOL, 0);
+          std::fill(have, have + NCOL, false);
+        }
+        if (self_ok)
+          for (int o = 0; o < NCOL; ++o)
+            if (nh[o] && !have[o]) {
+              cur[o] = nd[o];
+              have[o] = true;
+              dl_out[dl_out.size() - NCOL + o] = nd[o];
+            }
+        seg_of[n] = static_cast<int32_t>(dl_out.size() / NCOL - 1);  // within the range
+      }
+    });
+    std::vector<int64_t> seg_base(nranges + 1, 0);
+    for (int r = 0; r < nranges; ++r) seg_base[r + 1] = seg_base[r] + static_cast<int64_t>(part[r].size() / NCOL);
+    if (seg_base[nranges] >= (int64_t(1) << 31)) return fail(LBM_ERR_ARG, "too many column segments");
+    delta.reserve(static_cast<size_t>(seg_base[nranges]) * NCOL);
+    for (auto& v : part) {
+      delta.insert(delta.end(), v.begin(), v.end());
+      std::vector<int32_t>().swap(v);
     }
+    lbm::parallel_ranges(bounds, [&](int r, int64_t a, int64_t b) {
+      const int32_t add = static_cast<int32_t>(seg_base[r]);
+      for (int64_t n = a; n < b; ++n) seg_of[n] += add;
+    });
     if (delta.empty()) delta.assign(NCOL, 0);
   }
 
@@ -505,8 +544,8 @@ int create_columns(lbm_solver* s, const Tables& t) {
   // own column); missing links become bounce-back bits.
   std::vector<uint8_t> reg(N, 0);
   std::vector<uint32_t> bbm_node(N, 0);
-  for (int64_t n = 0; n < N; ++n) {
-    bool r = !((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b));
+  lbm::parallel_for(N, 0, [&](int64_t n) {
+    bool r = !bc_node(n);
     uint32_t bm = 0;
     for (int q = 0; q < Q && r; ++q) {
       const int32_t sq = srcq(q, n);
@@ -520,26 +559,24 @@ int create_columns(lbm_solver* s, const Tables& t) {
     }
     reg[n] = r ? 1 : 0;
     bbm_node[n] = r ? bm : 0;
-  }
+  });
 
   std::vector<int32_t> inlet_index(N, -1);
   for (size_t i = 0; i < t.inlet_nodes.size(); ++i) inlet_index[t.inlet_nodes[i]] = static_cast<int32_t>(i);
   std::vector<uint8_t> is_nw(N, 0);
   for (int32_t n : t.nearwall) is_nw[n] = 1;
 
-  // Chunk metadata and the compact per-kind arrays, all in slot order.
+  // Chunk metadata and the compact per-kind arrays, all in slot order: the
+  // masks and counts per chunk, their running sums as the chunks' bases,
+  // then the entries.
   const int64_t nchunk = npad / CHUNK;
   std::vector<ChunkMeta> meta(nchunk);
-  std::vector<uint32_t> bbm;
-  std::vector<int32_t> rows, nw_slot_order, bsl;
-  std::vector<int32_t> nw_rank(N, -1);
-  for (int64_t c = 0; c < nchunk; ++c) {
+  std::vector<int64_t> bsl_base(nchunk + 1, 0);
+  lbm::parallel_for(nchunk, 0, [&](int64_t c) {
     ChunkMeta& m = meta[c];
     std::memset(&m, 0, sizeof(m));
     m.seg_first = c * CHUNK < N ? seg_of[c * CHUNK] : 0;
-    m.bb_base = static_cast<int32_t>(bbm.size());
-    m.irr_base = static_cast<int32_t>(rows.size());
-    m.nw_base = static_cast<int32_t>(nw_slot_order.size());
+    int32_t nbs = 0;
     for (int l = 0; l < CHUNK; ++l) {
       const int64_t n = c * CHUNK + l;
       if (n >= N) break;
@@ -547,20 +584,46 @@ int create_columns(lbm_solver* s, const Tables& t) {
       if (l > 0 && seg_of[n] != seg_of[n - 1]) m.seg_mask |= bit;
       if (!reg[n]) {
         m.irr_mask |= bit;
-        rows.push_back(static_cast<int32_t>(n));
-        if ((t.kind[n] == KIND_INLET && inl_b) || (t.kind[n] == KIND_OUTLET && out_b))
-          bsl.push_back(static_cast<int32_t>(n));
+        if (bc_node(n)) ++nbs;
       } else if (bbm_node[n]) {
         m.bb_mask |= bit;
-        bbm.push_back(bbm_node[n]);
       }
-      if (is_nw[n]) {
-        m.nw_mask |= bit;
-        nw_rank[n] = static_cast<int32_t>(nw_slot_order.size());
-        nw_slot_order.push_back(static_cast<int32_t>(n));
+      if (is_nw[n]) m.nw_mask |= bit;
+    }
+    bsl_base[c + 1] = nbs;
+  });
+  int64_t n_rows = 0, n_bb = 0, n_nwo = 0;
+  for (int64_t c = 0; c < nchunk; ++c) {
+    ChunkMeta& m = meta[c];
+    m.irr_base = static_cast<int32_t>(n_rows);
+    m.bb_base = static_cast<int32_t>(n_bb);
+    m.nw_base = static_cast<int32_t>(n_nwo);
+    n_rows += __builtin_popcountll(m.irr_mask);
+    n_bb += __builtin_popcountll(m.bb_mask);
+    n_nwo += __builtin_popcountll(m.nw_mask);
+    bsl_base[c + 1] += bsl_base[c];
+  }
+  std::vector<uint32_t> bbm(n_bb);
+  std::vector<int32_t> rows(n_rows), nw_slot_order(n_nwo), bsl(bsl_base[nchunk]);
+  std::vector<int32_t> nw_rank(N, -1);
+  lbm::parallel_for(nchunk, 0, [&](int64_t c) {
+    const ChunkMeta& m = meta[c];
+    int64_t ir = m.irr_base, ib = m.bb_base, iw = m.nw_base, is = bsl_base[c];
+    for (int l = 0; l < CHUNK; ++l) {
+      const int64_t n = c * CHUNK + l;
+      if (n >= N) break;
+      if ((m.irr_mask >> l) & 1u) {
+        rows[ir++] = static_cast<int32_t>(n);
+        if (bc_node(n)) bsl[is++] = static_cast<int32_t>(n);
+      } else if ((m.bb_mask >> l) & 1u) {
+        bbm[ib++] = bbm_node[n];
+      }
+      if ((m.nw_mask >> l) & 1u) {
+        nw_rank[n] = static_cast<int32_t>(iw);
+        nw_slot_order[iw++] = static_cast<int32_t>(n);
       }
     }
-  }
+  });
   const int64_t nirr = static_cast<int64_t>(rows.size());
   const int64_t nnw = static_cast<int64_t>(nw_slot_order.size());
   s->n_irr = nirr;
@@ -570,13 +633,13 @@ int create_columns(lbm_solver* s, const Tables& t) {
   std::vector<int32_t> isrc(static_cast<size_t>(Q) * std::max<int64_t>(nirr, 1), -1);
   std::vector<uint32_t> ikind(std::max<int64_t>(nirr, 1), 0);
   std::vector<int32_t> ibcn(std::max<int64_t>(nirr, 1), -1), iinl(std::max<int64_t>(nirr, 1), -1);
-  for (int64_t r = 0; r < nirr; ++r) {
+  lbm::parallel_for(nirr, 0, [&](int64_t r) {
     const int32_t n = rows[r];
     for (int q = 0; q < Q; ++q) isrc[static_cast<size_t>(q) * nirr + r] = srcq(q, n);
     ikind[r] = t.kind[n];
     ibcn[r] = t.bc_nbr[n];
     iinl[r] = inlet_index[n];
-  }
+  });
   std::vector<double> nrm(3 * static_cast<size_t>(std::max<int64_t>(nnw, 1)), 0.0);
   for (size_t i = 0; i < t.nearwall.size(); ++i)
     for (int d = 0; d < 3; ++d)
@@ -584,7 +647,8 @@ int create_columns(lbm_solver* s, const Tables& t) {
 
   // Decode every slot the way the kernel does and compare with the int32
   // pull table (-1 = bounce back).
-  for (int64_t n = 0; n < N; ++n) {
+  std::atomic<int> bad{0};
+  lbm::parallel_for(N, 0, [&](int64_t n) {
     const ChunkMeta& m = meta[n / CHUNK];
     const int l = static_cast<int>(n % CHUNK);
     for (int q = 0; q < Q; ++q) {
@@ -597,10 +661,12 @@ int create_columns(lbm_solver* s, const Tables& t) {
         const int64_t base = o < 0 ? n : n + delta[static_cast<size_t>(segment_of(m, l)) * NCOL + o];
         got = ((bm >> q) & 1u) ? -1 : static_cast<int32_t>(base - L.cz[q]);
       }
-      if (got != srcq(q, n)) return fail(LBM_ERR_STATE, "column-segment encoding does not reproduce the neighbour table");
+      if (got != srcq(q, n)) bad.store(1, std::memory_order_relaxed);
     }
-    if (((m.nw_mask >> l) & 1u) != is_nw[n]) return fail(LBM_ERR_STATE, "near-wall mask mismatch");
-  }
+    if (((m.nw_mask >> l) & 1u) != is_nw[n]) bad.store(2, std::memory_order_relaxed);
+  });
+  if (bad.load() == 1) return fail(LBM_ERR_STATE, "column-segment encoding does not reproduce the neighbour table");
+  if (bad.load() == 2) return fail(LBM_ERR_STATE, "near-wall mask mismatch");
 
   int rc = LBM_OK;
   const size_t pop_bytes = static_cast<size_t>(Q) * npad * s->esize;

@@ -16,41 +16,13 @@
 #include "../lattice.h"
 #include "classify.h"
 #include "normals.h"
+#include "parallel.h"
 #include "raytest.h"
 #include "voxel.h"
 
 namespace lbm {
 
 namespace {
-
-int resolve_threads(int n) {
-  if (n > 0) return n;
-  unsigned h = std::thread::hardware_concurrency();
-  return h == 0 ? 1 : static_cast<int>(std::min(h, 64u));
-}
-
-// Static-chunked parallel loop over [0, n).
-template <class F>
-void parallel_for(int64_t n, int nthreads, F&& body) {
-  nthreads = resolve_threads(nthreads);
-  if (nthreads <= 1 || n < 2) {
-    for (int64_t i = 0; i < n; ++i) body(i);
-    return;
-  }
-  std::atomic<int64_t> next{0};
-  const int64_t grain = std::max<int64_t>(1, n / (int64_t(nthreads) * 16));
-  std::vector<std::thread> pool;
-  for (int t = 0; t < nthreads; ++t)
-    pool.emplace_back([&]() {
-      for (;;) {
-        const int64_t a = next.fetch_add(grain);
-        if (a >= n) break;
-        const int64_t b = std::min(n, a + grain);
-        for (int64_t i = a; i < b; ++i) body(i);
-      }
-    });
-  for (auto& th : pool) th.join();
-}
 
 // Ray end points of labelPoints case 0 plus adjustLineCoordinates case 0
 // (ComplexGeometry.cpp:314-336, 463-468).

@@ -25,19 +25,15 @@ def _num(v):
 def write_stl(path, tris, normals, name="vessel"):
     """tris: (T, 3, 3) vertices, normals: (T, 3). Written with 17 significant
     digits so that the reader recovers the exact doubles."""
-    tris = np.asarray(tris, dtype=np.float64)
+    tris = np.asarray(tris, dtype=np.float64).reshape(-1, 9)
     normals = np.asarray(normals, dtype=np.float64)
-    lines = [f"solid {name}"]
-    for t, n in zip(tris, normals):
-        lines.append(f"  facet normal {_num(n[0])} {_num(n[1])} {_num(n[2])}")
-        lines.append("    outer loop")
-        for v in t:
-            lines.append(f"      vertex {_num(v[0])} {_num(v[1])} {_num(v[2])}")
-        lines.append("    endloop")
-        lines.append("  endfacet")
-    lines.append(f"endsolid {name}")
+    facet = ("  facet normal %r %r %r\n    outer loop\n" + "      vertex %r %r %r\n" * 3 +
+             "    endloop\n  endfacet\n")
+    rows = np.concatenate([normals, tris], 1).tolist()
     with open(path, "w") as f:
-        f.write("\n".join(lines) + "\n")
+        f.write(f"solid {name}\n")
+        f.write("".join(facet % tuple(r) for r in rows))
+        f.write(f"endsolid {name}\n")
 
 
 def _unit(v):
@@ -491,27 +487,10 @@ def _catmull_rom(P, samples_per_seg):
     return np.vstack(out)
 
 
-def tortuous_vessel(scale=4.5, seed=5, device=None, sample_step=2.0, slab=32):
-    """Tortuous vessel of the large single-GPU share: a Catmull-Rom
-    centreline through a 12-point random walk (step 80 scale lu, a persistent
-    walk heading +z whose first and last steps run along z), radius varying
-    between 20 and 40 scale lu along the arc, and a spherical saccular bulge
-    of radius 60 scale lu on the side of the vessel at 55% of its length.
-    scale = 4.5 gives about 300M nodes at pitch 1 (the 80/20-40/60 lu sizes
-    as such give about 3.3M).
-
-    Labels are the union of balls of radius r(s) centred on the centreline
-    sampled every sample_step lu (a swept tube), and the bulge ball, on the
-    lattice x = origin + (i, j, k); the box is cut at the end points' z, so
-    the vessel's ends lie on the z faces (inlet at z min, normal -z; outlet
-    at z max, normal +z). Computed slab by slab with torch on `device`
-    (default: the GPU when there is one). Returns (fluid uint8 [nx, ny, nz],
-    origin, info); info holds the triangles of a swept-ring surface mesh and
-    the bulge sphere (for the near-wall normals), the plane blocks and sizes."""
-    import torch
-
-    if device is None:
-        device = "cuda" if torch.cuda.is_available() else "cpu"
+def _vessel_shape(scale, seed, sample_step):
+    """Centreline samples C (every sample_step lu of arc), radii R, the bulge
+    centre and radius, the walk's control points and the arc length of the
+    tortuous vessel (tortuous_vessel describes the shape)."""
     rng = np.random.default_rng(seed)
     step, rmin, rmax, rb = 80.0 * scale, 20.0 * scale, 40.0 * scale, 60.0 * scale
     P = [np.zeros(3), np.array([0.0, 0.0, step])]
@@ -538,6 +517,97 @@ def tortuous_vessel(scale=4.5, seed=5, device=None, sample_step=2.0, slab=32):
     tb = _unit(C[min(kb + 1, len(C) - 1)] - C[max(kb - 1, 0)])
     nb = _unit(np.cross(tb, rng.normal(size=3)))
     cb = C[kb] + nb * (R[kb] + 0.35 * rb)
+    return C, R, cb, rb, P, length
+
+
+def vessel_function(C, R, cb, rb, margin, band=8, device="cpu"):
+    """Implicit function of the tortuous vessel (negative inside): the union
+    of the balls (C[k], R[k]) and the bulge ball, cut by the planes through
+    the end points' z. Evaluated in float64 with torch on `device`, on
+    meshgrid slabs (indexing "ij") band by band of z levels, each against the
+    balls within `margin` lu of the band and of the slab's x range; a point
+    with no ball in reach gets a large positive value (it lies further than
+    `margin` from the surface)."""
+    import torch
+
+    z0, z1 = float(C[0, 2]), float(C[-1, 2])
+    big = 1e6
+    dev = torch.device(device)
+    Ct = torch.tensor(C, dtype=torch.float64, device=dev)
+    Rt = torch.tensor(R, dtype=torch.float64, device=dev)
+
+    def F(X, Y, Z):
+        xs, zs = X[:, 0, 0], Z[0, 0, :]
+        xa, xb = float(xs.min()), float(xs.max())
+        nearx = (C[:, 0] + R + margin >= xa) & (C[:, 0] - R - margin <= xb)
+        Xt, Yt, Zt = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (X, Y, Z))
+        out = torch.empty_like(Xt)
+        for k0 in range(0, len(zs), band):
+            k1 = min(len(zs), k0 + band)
+            za, zb = float(zs[k0:k1].min()), float(zs[k0:k1].max())
+            sel = np.where(nearx & (C[:, 2] + R + margin >= za) & (C[:, 2] - R - margin <= zb))[0]
+            Xb, Yb, Zb = Xt[:, :, k0:k1], Yt[:, :, k0:k1], Zt[:, :, k0:k1]
+            f = torch.sqrt((Xb - cb[0]) ** 2 + (Yb - cb[1]) ** 2 + (Zb - cb[2]) ** 2) - rb
+            for c0 in range(0, len(sel), 32):
+                idx = torch.as_tensor(sel[c0:c0 + 32], device=dev)
+                c, r = Ct[idx], Rt[idx]
+                d = torch.sqrt((Xb[..., None] - c[:, 0]) ** 2 + (Yb[..., None] - c[:, 1]) ** 2 +
+                               (Zb[..., None] - c[:, 2]) ** 2) - r
+                f = torch.minimum(f, d.amin(-1))
+            f = torch.clamp(f, max=big)
+            f = torch.maximum(f, z0 - Zb)
+            out[:, :, k0:k1] = torch.maximum(f, Zb - z1)
+        return out.cpu().numpy()
+
+    return F
+
+
+def tortuous_vessel_stl(workdir, scale=4.3, seed=5, sample_step=2.0, mesh_h=8.0, tau=0.8, max_iter=1000,
+                        device="cpu"):
+    """The tortuous vessel (tortuous_vessel) as a closed STL surface and a
+    config.txt, for the set-up chain from the surface: the zero level set of
+    vessel_function (on `device`) by marching tetrahedra at pitch mesh_h,
+    written with 17 digits; grid_size 0.5 (a lattice pitch of 1: the reader doubles it). Inlet: the z min cap (normal -z); outlet: the last
+    grid layer inside the z max cap (normal +z). Returns (config path, info)."""
+    os.makedirs(workdir, exist_ok=True)
+    C, R, cb, rb, _, length = _vessel_shape(scale, seed, sample_step)
+    lo = np.minimum((C - R[:, None]).min(0), cb - rb)
+    hi = np.maximum((C + R[:, None]).max(0), cb + rb)
+    tris, nrm = marching_tetrahedra(vessel_function(C, R, cb, rb, 3.0 * mesh_h, device=device), lo, hi, mesh_h)
+    stl = os.path.join(workdir, "vessel.stl")
+    write_stl(stl, tris, nrm, "vessel")
+    zmin, zmax = float(tris[..., 2].min()), float(tris[..., 2].max())
+    z_out = zmin + math.floor(zmax - zmin)
+    cfg = os.path.join(workdir, "config.txt")
+    write_config(cfg, "vessel.stl", tau=tau, max_iter=max_iter, grid_size=0.5,
+                 inlets=[dict(plane="z", coordinate=zmin, normal=(0, 0, -1))],
+                 outlets=[dict(plane="z", coordinate=z_out, normal=(0, 0, 1))])
+    return cfg, dict(scale=scale, length=length, centreline=C, radius=R, bulge_center=cb, bulge_radius=rb,
+                     n_triangles=len(tris), z_in=zmin, z_out=z_out)
+
+
+def tortuous_vessel(scale=4.5, seed=5, device=None, sample_step=2.0, slab=32):
+    """Tortuous vessel of the large single-GPU share: a Catmull-Rom
+    centreline through a 12-point random walk (step 80 scale lu, a persistent
+    walk heading +z whose first and last steps run along z), radius varying
+    between 20 and 40 scale lu along the arc, and a spherical saccular bulge
+    of radius 60 scale lu on the side of the vessel at 55% of its length.
+    scale = 4.5 gives about 300M nodes at pitch 1 (the 80/20-40/60 lu sizes
+    as such give about 3.3M).
+
+    Labels are the union of balls of radius r(s) centred on the centreline
+    sampled every sample_step lu (a swept tube), and the bulge ball, on the
+    lattice x = origin + (i, j, k); the box is cut at the end points' z, so
+    the vessel's ends lie on the z faces (inlet at z min, normal -z; outlet
+    at z max, normal +z). Computed slab by slab with torch on `device`
+    (default: the GPU when there is one). Returns (fluid uint8 [nx, ny, nz],
+    origin, info); info holds the triangles of a swept-ring surface mesh and
+    the bulge sphere (for the near-wall normals), the plane blocks and sizes."""
+    import torch
+
+    if device is None:
+        device = "cuda" if torch.cuda.is_available() else "cpu"
+    C, R, cb, rb, P, length = _vessel_shape(scale, seed, sample_step)
     z0, z1 = float(C[0, 2]), float(C[-1, 2])
     lo = np.minimum((C - R[:, None]).min(0), cb - rb) - 2.0
     hi = np.maximum((C + R[:, None]).max(0), cb + rb) + 2.0

@@ -9,9 +9,9 @@
   random porosity direction, to 1e-3; the FP32 adjoint against the FP64 one,
   to 1e-3 in relative L2.
 * The ~300M-node tortuous-vessel share (its state exceeds 4 GiB, so the step
-  takes 64-bit offsets): its tables classified on the GPU (classify_device)
-  equal the host chain's (128-bit digests of every table), then 100 steps in
-  FP32 and FP64. The vessel is closed
+  takes 64-bit offsets), built from its STL: the set-up chain with ray
+  labels, classification and tables on the GPU gives the host chain's labels
+  and tables (128-bit digests of each), then 100 steps in FP32 and FP64. The vessel is closed
   (its end nodes bounce back, LBM_INLET_NONE / LBM_OUTLET_NONE) and driven by
   a body force along z, so its mass is conserved exactly and the FP64 mass
   sum may drift by rounding only (1e-10 relative); FP32 storage against FP64
@@ -25,7 +25,6 @@ import numpy as np
 import pytest
 
 import lbm
-import workloads
 from conftest import ROOT
 
 pytestmark = pytest.mark.gpu
@@ -101,9 +100,9 @@ def test_adjoint_on_the_bifurcation(bifurcation):
 
 
 def _large_share_child():
-    """The large-share comparison, run in a process of its own: torch (which
-    casts the vessel's labels on the GPU) initialises its device before the
-    library loads, and the ~30 GB of host tables go with the process."""
+    """The large-share comparison, run in a process of its own: torch (the
+    free-memory check) initialises its device before the library loads, and
+    the ~30 GB of host tables go with the process."""
     import gc
     import json
 
@@ -114,21 +113,12 @@ def _large_share_child():
     if free < 160e9:
         print(json.dumps({"skip": f"needs ~160 GB of free device memory, {free / 1e9:.0f} GB free"}))
         return
-    fluid, origin, info = workloads.tortuous_vessel(scale=bench.LARGE_SCALE)
-    nz = fluid.shape[2]
-    inl = lbm.plane_block("z", float(origin[2]), (0.0, 0.0, -1.0))
-    out_b = lbm.plane_block("z", float(origin[2] + nz - 1), (0.0, 0.0, 1.0))
-
-    def build(on_device):
-        return lbm.Geometry.from_labels(fluid, origin, 1.0, inlets=[inl], outlets=[out_b],
-                                        triangles=info["triangles"], normals=2,
-                                        nthreads=int(os.environ.get("OMP_NUM_THREADS", "16")),
-                                        classify_device=on_device)
 
     def digest(g):
         out = [tuple(getattr(g.info, k) for k in ("n_nodes", "n_wall", "n_inlet", "n_outlet", "n_nearwall",
                                                    "n_boundary_points", "upstream_out_of_box"))]
-        for get in (g.nodes, lambda: (g.neighbours(),), lambda: (g.bc_neighbour(),), g.nearwall):
+        for get in (lambda: (g.labels(),), g.nodes, lambda: (g.neighbours(),), lambda: (g.bc_neighbour(),),
+                    g.nearwall):
             h = xxhash.xxh3_128()
             for a in get():
                 h.update(np.ascontiguousarray(a).view(np.uint8).reshape(-1))
@@ -136,14 +126,14 @@ def _large_share_child():
             gc.collect()
         return out
 
-    host = build(False)
+    host, _, _ = bench.large_share_geometry(host_chain=True)
     host_digest = digest(host)
     host.close()
     del host
     gc.collect()
-    geom = build(True)
-    del fluid
-    print(f"tortuous vessel: {geom.n} nodes, {geom.info.n_nearwall} near-wall", file=sys.stderr, flush=True)
+    geom, _, setup = bench.large_share_geometry()
+    print(f"tortuous vessel: {geom.n} nodes, {geom.info.n_nearwall} near-wall, set-up {setup}", file=sys.stderr,
+          flush=True)
     res = {"nodes": geom.n, "device_tables_equal_host": digest(geom) == host_digest}
     steps, g = 100, 2e-6
     sample = np.sort(np.random.default_rng(11).choice(geom.n, 2_000_000, replace=False)).astype(np.int32)

@@ -51,3 +51,23 @@ def test_vessel_geometry_builds_with_outward_normals(vessel):
     radial[in_bulge] = (p - info["bulge_center"])[in_bulge]
     cosang = (radial * nrm).sum(1) / np.maximum(np.linalg.norm(radial, axis=1) * np.linalg.norm(nrm, axis=1), 1e-12)
     assert np.mean(cosang > 0.5) > 0.95
+
+
+def test_vessel_stl_is_closed_and_matches_the_labels(vessel, tmp_path):
+    """The vessel's STL (the large share's surface, here at scale 0.5): every
+    edge is shared by exactly two facets (marching tetrahedra interpolate a
+    shared edge once, so the copies are bit-identical), and the set-up chain
+    from it labels nearly the same points as the ball union."""
+    fluid, _, _ = vessel
+    cfg, info = workloads.tortuous_vessel_stl(str(tmp_path), scale=0.5, mesh_h=2.0)
+    g = lbm.Geometry.from_config(cfg, normals=0)
+    tris = g.triangles()[:, :9].reshape(-1, 3, 3)  # 3 vertices, then the file normal
+    assert len(tris) == info["n_triangles"]
+    v, inv = np.unique(tris.reshape(-1, 3), axis=0, return_inverse=True)
+    t = inv.reshape(-1, 3)
+    e = np.sort(np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]]), axis=1)
+    _, count = np.unique(e, axis=0, return_counts=True)
+    assert (count == 2).all()
+    n = int(fluid.sum())
+    assert abs(g.n - n) <= 0.01 * n, (g.n, n)
+    assert g.info.n_inlet > 0 and g.info.n_outlet > 0

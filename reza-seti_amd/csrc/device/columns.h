@@ -17,8 +17,9 @@
 // compact array by a population count over the lower lanes.
 //
 // One launch per step: the first `b_blocks` blocks handle inlet/outlet nodes
-// (one thread each), the rest every other slot, XCD-remapped. Near-wall
-// nodes accumulate the wall shear stress inside the same launch.
+// (one thread each), the rest every other slot, XCD-remapped (each XCD walks
+// every eighth run of xcd_run blocks). Near-wall nodes accumulate the wall
+// shear stress inside the same launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -84,6 +85,7 @@ struct ColArgs {
   int32_t xcd_remap;
   int32_t xcd_first[8];
   int32_t xcd_offset[8];
+  int32_t xcd_run;       // > 0: XCD x takes runs x, x + 8, x + 16, ... of xcd_run main blocks
   const ChunkMeta* meta;     // npad / CHUNK
   const int32_t* seg_delta;  // NCOL per segment (32-byte aligned rows)
   const uint32_t* bbm;       // missing-link masks (bit q), bb_mask order
@@ -133,6 +135,26 @@ struct ColArgs {
   const uint8_t* spec_flag;
   int32_t wide;  // 1: 64-bit element offsets whatever the state size (the path of states of 4 GiB or more)
 };
+
+// Blocks of the main part of a launch: m_blocks, rounded up to whole rounds
+// of 8 runs when the XCD runs apply.
+inline int64_t main_grid_blocks(const ColArgs& a) {
+  if (!a.xcd_remap || a.xcd_run <= 0) return a.m_blocks;
+  const int64_t round = 8 * int64_t(a.xcd_run);
+  return (a.m_blocks + round - 1) / round * round;
+}
+
+// Main block that grid block b (>= b_blocks) takes, or -1 for a block of the
+// rounding. The hardware hands blocks to the 8 XCDs in turn (block b to XCD
+// b mod 8); k counts XCD x's blocks in dispatch order.
+__device__ inline int32_t main_block_of(const ColArgs& a, int32_t b) {
+  if (!a.xcd_remap) return b - a.b_blocks;
+  const int x = b & 7;
+  const int32_t k = (b - a.xcd_first[x]) / 8;
+  if (a.xcd_run <= 0) return a.xcd_offset[x] + k;
+  const int32_t lb = (k / a.xcd_run) * 8 * a.xcd_run + x * a.xcd_run + k % a.xcd_run;
+  return lb < a.m_blocks ? lb : -1;
+}
 
 hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st);
 // Adds ring slots 0 .. count-1 (in order) to the FP64 accumulators and sets

@@ -33,11 +33,8 @@ __device__ inline void columns_body(const ColArgs& a) {
     if (MODE != KMODE_INIT && i < a.n_bslots) ColKern<L, S>::template boundary_slot<MODE, FORCE, NT>(a, a.bslots[i], i);
     return;
   }
-  int32_t lb = b - a.b_blocks;
-  if (a.xcd_remap) {
-    const int x = b & 7;
-    lb = a.xcd_offset[x] + (b - a.xcd_first[x]) / 8;
-  }
+  const int32_t lb = main_block_of(a, b);
+  if (lb < 0) return;
   const int32_t p = lb * BLOCK + threadIdx.x;
   // One wavefront covers one chunk: its metadata is wave-uniform.
   const int32_t chunk = __builtin_amdgcn_readfirstlane(p / CHUNK);
@@ -123,7 +120,7 @@ __global__ __launch_bounds__(BLOCK) void wss_stats_cols(ColArgs a, double* parti
 
 template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32 = false, int KRES = 0>
 hipError_t launch_one(const ColArgs& a, hipStream_t st) {
-  const unsigned grid = static_cast<unsigned>(a.b_blocks + a.m_blocks);
+  const unsigned grid = static_cast<unsigned>(a.b_blocks + main_grid_blocks(a));
   if (!grid) return hipSuccess;
 #if LBM_COLUMNS_WAVES_PER_EU > 0
   if constexpr (!std::is_same<S, double>::value) {

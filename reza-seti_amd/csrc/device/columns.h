@@ -138,7 +138,7 @@ struct ColArgs {
 
 // Blocks of the main part of a launch: m_blocks, rounded up to whole rounds
 // of 8 runs when the XCD runs apply.
-inline int64_t main_grid_blocks(const ColArgs& a) {
+__host__ __device__ inline int64_t main_grid_blocks(const ColArgs& a) {
   if (!a.xcd_remap || a.xcd_run <= 0) return a.m_blocks;
   const int64_t round = 8 * int64_t(a.xcd_run);
   return (a.m_blocks + round - 1) / round * round;

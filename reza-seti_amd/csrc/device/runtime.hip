@@ -724,14 +724,6 @@ int create_columns(lbm_solver* s, const Tables& t) {
   c.m_blocks = static_cast<int32_t>(nblk);
   c.xcd_remap = 1;
   xcd_tables(c.b_blocks, c.m_blocks, c.xcd_first, c.xcd_offset);
-  // XCD x walks runs x, x + 8, x + 16, ... of 64 blocks (16,384 slots)
-  // rather than one contiguous eighth: neighbours along y and z stay in the
-  // run (its XCD's L2), and every XCD gets a like share of the geometry, so
-  // the XCDs finish together (on the 2.85M-node bifurcation the eighths
-  // holding the inclined daughter branches ran 20% longer than the middle
-  // ones; 77.5 -> 75.9 us per FP32 step; runs of 16 or 256 blocks: 77.0,
-  // 79.1 us).
-  c.xcd_run = 64;
   c.meta = s->meta;
   c.seg_delta = s->seg_delta;
   c.bbm = s->bbm;
@@ -782,6 +774,15 @@ int create_columns(lbm_solver* s, const Tables& t) {
 #ifdef LBM_CACHE_POLICY
   s->cache_policy = LBM_CACHE_POLICY;  // tuning builds only
 #endif
+  // XCD x walks runs x, x + 8, x + 16, ... of 64 blocks (16,384 slots)
+  // rather than one contiguous eighth: neighbours along y and z stay in the
+  // run (its XCD's L2), and every XCD gets a like share of the geometry, so
+  // the XCDs finish together (on the 2.85M-node bifurcation the eighths
+  // holding the inclined daughter branches ran 20% longer than the middle
+  // ones: FP32 step 86.8 -> 82.1 us, FP64 +6.6%; runs of 16 or 256 blocks
+  // gained less or lost). A state whose two buffers both fit the Infinity
+  // Cache keeps the contiguous eighths (FP16 on that vessel: 2% faster).
+  c.xcd_run = s->cache_policy == 0 ? 0 : 64;
   c.cache_hint = CACHE_PLAIN;
 
   // Generic fields used by the shared moments kernel and init_state.

@@ -240,6 +240,9 @@ struct AdjKern {
     const bool nw = (m.nw_mask >> l) & 1u;
     const int32_t i = nw ? m.nw_base + rank_below(m.nw_mask, l) : -1;
     const bool in_obj = nw && a.obj_w && a.obj_w[i] != 0.0;
+    // dJ/dalpha is read and written once per reverse step: streamed, so that
+    // it does not take Infinity Cache room from the adjoint field.
+    const double g_old = a.g_alpha ? __builtin_nontemporal_load(a.g_alpha + p) : 0.0;
     // Moments of the step's pre-collision populations G_t: from the tape of
     // the recomputation, or recomputed here.
     C s, jx, jy, jz;
@@ -282,7 +285,7 @@ struct AdjKern {
       }
       dal += wss_adjoint(a, p, i, e, tx, ty, tz, lam);
     }
-    if (a.g_alpha) a.g_alpha[p] += dal;
+    if (a.g_alpha) __builtin_nontemporal_store(g_old + dal, a.g_alpha + p);
     if (a.mode == ADJ_FINAL) return;
 #pragma unroll
     for (int q = 0; q < Q; ++q) ast_h<NTOUT>(a.rho_out, int64_t(q) * c.npad + p, lam[q]);

@@ -270,7 +270,10 @@ int lbm_get_populations(lbm_solver* s, double* f);
 /* Post-collision populations of the next iteration (m_newDistributions of the
  * collision that follows), FP64, same layout. */
 int lbm_get_post_collision(lbm_solver* s, double* fs);
-/* Overwrite f(t) of every node (Node::m_distributions) and redo the collision. */
+/* Overwrite f(t) of every node (Node::m_distributions) and redo the collision.
+ * f = NULL sets the rest state f = w_q that lbm_create starts from (no host
+ * copy of the state is kept, so a following adjoint pass starts from w_q as
+ * after lbm_create). */
 int lbm_set_populations(lbm_solver* s, const double* f);
 
 /* Near-wall WSS in lbm_geometry_get_nearwall order: instantaneous |tau|

@@ -372,6 +372,11 @@ struct AdjKern {
 #pragma clang fp contract(fast)
     const ColArgs& c = a.c;
     const int32_t y = a.tgt[k];
+    // rho(y) as kernel A wrote it, loaded first: its round trip overlaps the
+    // member loads below instead of following them.
+    C old[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) old[q] = ald(a.rho_out, int64_t(q) * c.npad + y);
     C s, jx, jy, jz;
     if (a.tape_in) {  // y is a main slot: its pulled moments are on the tape
       const C* t = static_cast<const C*>(a.tape_in) + y;
@@ -423,10 +428,7 @@ struct AdjKern {
         acc[q] += mu[q] + cst + fac * (cdot<L, C>(q, gx, gy, gz) - gu);
     }
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int64_t idx = int64_t(q) * c.npad + y;
-      ast(a.rho_out, idx, ald(a.rho_out, idx) + acc[q]);
-    }
+    for (int q = 0; q < Q; ++q) ast(a.rho_out, int64_t(q) * c.npad + y, old[q] + acc[q]);
   }
 };
 

@@ -69,6 +69,7 @@ struct StepArgs {
   int32_t outlet_mode;
   double outlet_rho;
   int32_t zero_check;
+  int32_t nt_stores;  // 1: the FP64 step stores populations nontemporally (state beyond the Infinity Cache)
   uint32_t* err_flags;
   // KMODE_INIT input / KMODE_MATERIALIZE output: FP64 unshifted f, q * npad + slot.
   const double* f_direct;

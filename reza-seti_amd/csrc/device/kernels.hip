@@ -116,7 +116,7 @@ struct Kern {
     a.wss[4 * sp + p] += W;
   }
 
-  template <int MODE, bool FORCE>
+  template <int MODE, bool FORCE, bool NT = false>
   __device__ static void collide_store(const StepArgs& a, int32_t p, C* F, bool wall_zero_u, bool nearwall,
                                        bool special) {
     C drho, ux, uy, uz;
@@ -140,7 +140,10 @@ struct Kern {
       a.rho_u[3 * sp + p] = double(uz);
     }
 #pragma unroll
-    for (int q = 0; q < Q; ++q) P::st(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+    for (int q = 0; q < Q; ++q) {
+      if constexpr (NT) P::st_nt(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+      else P::st(a.fs_out, int64_t(q) * a.npad + p, F[q]);
+    }
   }
 
   __device__ static void load_direct(const StepArgs& a, int32_t p, C* F) {
@@ -156,7 +159,9 @@ struct Kern {
     for (int q = 0; q < Q; ++q) a.f_out[int64_t(q) * a.npad + p] = SHIFT ? double(F[q]) + L::w[q] : double(F[q]);
   }
 
-  template <int MODE, bool FORCE>
+  // NT: nontemporal population stores (a state larger than the Infinity
+  // Cache is streamed through it).
+  template <int MODE, bool FORCE, bool NT = false>
   __device__ static void special_node(const StepArgs& a, int32_t p) {
     const uint32_t fl = a.sflags[p];
     if (fl & SFLAG_PAD) return;
@@ -222,10 +227,10 @@ struct Kern {
       for (int q = 0; q < Q; ++q) a.fprev_out[int64_t(q) * a.n_special_pad + p] = double(F[q]);
     }
     const bool zero_u = (kind == 1) && (a.wall_mode == WALL_REF);
-    collide_store<MODE, FORCE>(a, p, F, zero_u, nearwall, true);
+    collide_store<MODE, FORCE, NT>(a, p, F, zero_u, nearwall, true);
   }
 
-  template <int MODE, bool FORCE>
+  template <int MODE, bool FORCE, bool NT = false>
   __device__ static void bulk_node(const StepArgs& a, int32_t p) {
     C F[Q];
     if (MODE == KMODE_INIT) {
@@ -243,7 +248,7 @@ struct Kern {
       write_direct(a, p, F);
       return;
     }
-    collide_store<MODE, FORCE>(a, p, F, false, false, false);
+    collide_store<MODE, FORCE, NT>(a, p, F, false, false, false);
   }
 };
 
@@ -377,37 +382,47 @@ __global__ __launch_bounds__(BLOCK) void finish_kernel(const double* partials, i
   }
 }
 
-// The step as two launches: the special slots (boundary rules, stress,
-// previous values; many registers) and the bulk slots (pull, collide, store;
-// few registers, so many more wavefronts in flight). Together they write what
-// one lbm_kernel launch writes.
-template <class L, class S, int MODE, bool FORCE>
+// The two launches of the FP64 step (launch_split); together they write
+// what one lbm_kernel launch writes.
+template <class L, class S, int MODE, bool FORCE, bool NT>
 __global__ __launch_bounds__(BLOCK) void lbm_special_kernel(StepArgs a) {
   const int32_t p = blockIdx.x * BLOCK + threadIdx.x;
-  if (p < a.n_special_pad) Kern<L, S>::template special_node<MODE, FORCE>(a, p);
+  if (p < a.n_special_pad) Kern<L, S>::template special_node<MODE, FORCE, NT>(a, p);
 }
 
-template <class L, class S, int MODE, bool FORCE>
+template <class L, class S, int MODE, bool FORCE, bool NT>
 __global__ __launch_bounds__(BLOCK) void lbm_bulk_kernel(StepArgs a) {
   const int32_t b = blockIdx.x;
   const int x = b & 7;
   const int32_t chunk = a.xcd_remap ? a.xcdb_offset[x] + (b - x) / 8 : b;
   const int32_t p = a.n_special_pad + chunk * BLOCK + threadIdx.x;
-  if (p < a.bulk_end) Kern<L, S>::template bulk_node<MODE, FORCE>(a, p);
+  if (p < a.bulk_end) Kern<L, S>::template bulk_node<MODE, FORCE, NT>(a, p);
 }
 
 #ifndef LBM_REF_SPLIT
 #define LBM_REF_SPLIT 1
 #endif
 
+// The FP64 step as two launches: the special slots (boundary rules, stress,
+// previous values; many registers) and the bulk slots (pull, collide, store;
+// few registers, so many more wavefronts in flight). (One launch per special
+// group, each with only its own rule's registers, took 419 us per step on the
+// 2.85M-node bifurcation against 355 us: the inlet/outlet group alone is a
+// 51 us chain of dependent loads, which the combined launch hides.)
+template <class L, class S, int MODE, bool FORCE, bool NT>
+hipError_t launch_split(const StepArgs& a, hipStream_t st) {
+  if (a.blocks_special)
+    hipLaunchKernelGGL((lbm_special_kernel<L, S, MODE, FORCE, NT>), dim3(a.blocks_special), dim3(BLOCK), 0, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !a.blocks_bulk) return e;
+  hipLaunchKernelGGL((lbm_bulk_kernel<L, S, MODE, FORCE, NT>), dim3(a.blocks_bulk), dim3(BLOCK), 0, st, a);
+  return hipGetLastError();
+}
+
 template <class L, class S, int MODE, bool FORCE>
 hipError_t launch_one(const StepArgs& a, hipStream_t st) {
   if constexpr (LBM_REF_SPLIT && MODE == KMODE_STEP && std::is_same<S, double>::value) {
-    if (a.blocks_special) hipLaunchKernelGGL((lbm_special_kernel<L, S, MODE, FORCE>), dim3(a.blocks_special), dim3(BLOCK), 0, st, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !a.blocks_bulk) return e;
-    hipLaunchKernelGGL((lbm_bulk_kernel<L, S, MODE, FORCE>), dim3(a.blocks_bulk), dim3(BLOCK), 0, st, a);
-    return hipGetLastError();
+    return a.nt_stores ? launch_split<L, S, MODE, FORCE, true>(a, st) : launch_split<L, S, MODE, FORCE, false>(a, st);
   }
   const dim3 grid(a.blocks_special + a.blocks_bulk);
   if (grid.x == 0) return hipSuccess;

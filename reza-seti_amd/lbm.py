@@ -479,7 +479,11 @@ class Solver:
         _check(lib().lbm_get_post_collision(self._h, _ptr(f)))
         return f
 
-    def set_populations(self, f):
+    def set_populations(self, f=None):
+        """f: (n, q) FP64 populations; None = the rest state f = w_q."""
+        if f is None:
+            _check(lib().lbm_set_populations(self._h, None))
+            return
         f = np.ascontiguousarray(f, dtype=np.float64)
         assert f.shape == (self.n, self.q)
         _check(lib().lbm_set_populations(self._h, _ptr(f)))

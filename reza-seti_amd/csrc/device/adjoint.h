@@ -45,6 +45,7 @@ struct AdjArgs {
   double inv_T;
   double dp_in, dp_out;  // a_p / (3 T n_in), a_p / (3 T n_out)
   const double* obj_w;   // n_nw: stress weight (0 = outside the wall segment)
+  const uint64_t* obj_mask;  // npad / CHUNK: bit l of chunk c set = slot c * CHUNK + l has a nonzero weight
   const double* g0;      // ADJ_FINAL: initial populations (FP64 unshifted, q * npad + slot), null = w_q
   // Kernel B: upstream nodes of extrapolated boundary nodes (CSR), with
   // everything the transposed rules need resolved on the host.

@@ -229,8 +229,9 @@ struct AdjKern {
     else K::template pull_regular<true>(a.c, p, K::load_deltas(a.c, seg), bm, G);
   }
 
+  // om: the chunk's objective mask (wave-uniform).
   template <bool NTIN, bool NTOUT>
-  __device__ static void main_slot(const AdjArgs& a, int32_t p, int l, const ChunkMeta& m) {
+  __device__ static void main_slot(const AdjArgs& a, int32_t p, int l, const ChunkMeta& m, uint64_t om) {
     const ColArgs& c = a.c;
     const bool irr = (m.irr_mask >> l) & 1u;
     const int32_t row = irr ? m.irr_base + rank_below(m.irr_mask, l) : -1;
@@ -239,7 +240,7 @@ struct AdjKern {
     const int32_t seg = segment_of(m, l);
     const bool nw = (m.nw_mask >> l) & 1u;
     const int32_t i = nw ? m.nw_base + rank_below(m.nw_mask, l) : -1;
-    const bool in_obj = nw && a.obj_w && a.obj_w[i] != 0.0;
+    const bool in_obj = (om >> l) & 1u;  // (objective nodes are near-wall nodes)
     // dJ/dalpha is read and written once per reverse step: streamed, so that
     // it does not take Infinity Cache room from the adjoint field.
     const double g_old = a.g_alpha ? __builtin_nontemporal_load(a.g_alpha + p) : 0.0;
@@ -265,6 +266,18 @@ struct AdjKern {
       }
       moments<L, C>(G, s, jx, jy, jz);
     }
+    // The objective's stress at this node, from the recomputed populations,
+    // before the adjoint field is loaded: the two 19-value arrays are never
+    // live together, which keeps this rare path within the register count
+    // of the common one (and so the whole launch at more waves per SIMD).
+    typename K::Eq e{};
+    C tx = C(0), ty = C(0), tz = C(0);
+    if (a.mode == ADJ_STEP && in_obj) {
+      e = K::template wss_equilibrium<true>(c, p);
+      C G[Q];
+      pull_main(a, p, irr, row, seg, bm, G);
+      tau_raw(c, i, e, G, tx, ty, tz);
+    }
     C lam[Q];
     if (a.rho_in) {
       if (irr) rpull_irregular<NTIN>(a, p, row, lam);
@@ -275,16 +288,7 @@ struct AdjKern {
       for (int q = 0; q < Q; ++q) lam[q] = C(0);
     }
     double dal = collide_adjoint(a, p, s, jx, jy, jz, lam);  // lam -> mu
-    if (a.mode == ADJ_STEP && in_obj) {
-      const auto e = K::template wss_equilibrium<true>(c, p);
-      C tx, ty, tz;
-      {
-        C G[Q];
-        pull_main(a, p, irr, row, seg, bm, G);
-        tau_raw(c, i, e, G, tx, ty, tz);
-      }
-      dal += wss_adjoint(a, p, i, e, tx, ty, tz, lam);
-    }
+    if (a.mode == ADJ_STEP && in_obj) dal += wss_adjoint(a, p, i, e, tx, ty, tz, lam);
     if (a.g_alpha) __builtin_nontemporal_store(g_old + dal, a.g_alpha + p);
     if (a.mode == ADJ_FINAL) return;
 #pragma unroll
@@ -321,11 +325,19 @@ struct AdjKern {
     const int32_t i = nw ? m.nw_base + rank_below(m.nw_mask, l) : -1;
     const bool in_obj = nw && a.obj_w && a.obj_w[i] != 0.0;
     const bool nee = u >= 0 && ((kind == 2 && c.inlet_mode == INLET_NEE) || (kind == 3 && c.outlet_mode == OUTLET_NEE));
+    // Moments and (at an objective node) the stress of the post-rule
+    // populations first, then the adjoint field (as in main_slot).
     C s, jx, jy, jz;
+    typename K::Eq e{};
+    C tx = C(0), ty = C(0), tz = C(0);
     {
       C G[Q];
       boundary_G(c, p, row, kind, u, nee, G);
       moments<L, C>(G, s, jx, jy, jz);
+      if (in_obj) {
+        e = K::template wss_equilibrium<true>(c, p);
+        tau_raw(c, i, e, G, tx, ty, tz);
+      }
     }
     C lam[Q];
     if (a.rho_in) {
@@ -336,16 +348,7 @@ struct AdjKern {
       for (int q = 0; q < Q; ++q) lam[q] = C(0);
     }
     double dal = collide_adjoint(a, p, s, jx, jy, jz, lam);  // lam -> mu
-    if (in_obj) {
-      const auto e = K::template wss_equilibrium<true>(c, p);
-      C tx, ty, tz;
-      {
-        C G[Q];
-        boundary_G(c, p, row, kind, u, nee, G);
-        tau_raw(c, i, e, G, tx, ty, tz);
-      }
-      dal += wss_adjoint(a, p, i, e, tx, ty, tz, lam);
-    }
+    if (in_obj) dal += wss_adjoint(a, p, i, e, tx, ty, tz, lam);
     const C dp = kind == 2 ? C(a.dp_in) : C(-a.dp_out);
 #pragma unroll
     for (int q = 0; q < Q; ++q) lam[q] += dp;
@@ -446,9 +449,10 @@ __device__ inline void adjoint_body(const AdjArgs& a) {
   const int32_t p = lb * BLOCK + threadIdx.x;
   const int32_t chunk = __builtin_amdgcn_readfirstlane(p / CHUNK);
   const ChunkMeta m = c.meta[chunk];
+  const uint64_t om = a.obj_mask ? a.obj_mask[chunk] : 0;
   if (p >= c.n) return;
-  if (a.rho_cached) AdjKern<L, S>::template main_slot<false, true>(a, p, p & (CHUNK - 1), m);
-  else AdjKern<L, S>::template main_slot<true, false>(a, p, p & (CHUNK - 1), m);
+  if (a.rho_cached) AdjKern<L, S>::template main_slot<false, true>(a, p, p & (CHUNK - 1), m, om);
+  else AdjKern<L, S>::template main_slot<true, false>(a, p, p & (CHUNK - 1), m, om);
 }
 
 template <class L, class S>

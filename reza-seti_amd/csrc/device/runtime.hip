@@ -187,6 +187,7 @@ struct lbm_solver {
     double* obj_w = nullptr;          // n_nw
     double* obj_acc = nullptr;        // n_nw
     double* obj_rho = nullptr;        // n_bslots
+    uint64_t* obj_mask = nullptr;     // npad / CHUNK: bit l of chunk c marks slot c * CHUNK + l in obj_slots
     std::vector<int32_t> obj_slots;   // slots of the near-wall nodes with a nonzero stress weight
   } adj;
 };
@@ -212,7 +213,7 @@ int dalloc_raw(lbm_solver* s, void** p, size_t bytes) {
 
 void adjoint_release(lbm_solver* s) {
   // The checkpoints live in the solver's persistent arena.
-  void* ptrs[] = {s->adj.obj_w, s->adj.obj_acc, s->adj.obj_rho};
+  void* ptrs[] = {s->adj.obj_w, s->adj.obj_acc, s->adj.obj_rho, s->adj.obj_mask};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   s->adj = lbm_solver::Adjoint{};
@@ -1468,9 +1469,16 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
   A.obj_slots.clear();
   for (size_t i = 0; i < s->nw_rows.size(); ++i)
     if (w[s->nw_rows[i]] != 0.0) A.obj_slots.push_back(s->nw_slots[i]);
+  // The same set as one bit per slot, read once per wavefront by the reverse
+  // step (a per-lane weight load would put a dependent memory round trip in
+  // front of every near-wall node's reverse pull).
+  std::vector<uint64_t> omask(static_cast<size_t>(s->npad / CHUNK), 0);
+  for (int32_t slot : A.obj_slots) omask[slot / CHUNK] |= uint64_t(1) << (slot % CHUNK);
   HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_w), nnw * 8));
   HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_acc), nnw * 8));
   HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_rho), nb * 8));
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&A.obj_mask), omask.size() * 8));
+  HIP_TRY(hipMemcpyAsync(A.obj_mask, omask.data(), omask.size() * 8, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(A.obj_w, w.data(), nnw * 8, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipMemsetAsync(A.obj_acc, 0, nnw * 8, s->stream));
   HIP_TRY(hipMemsetAsync(A.obj_rho, 0, nb * 8, s->stream));
@@ -1725,6 +1733,7 @@ int lbm_adjoint_backward(lbm_solver* s, int32_t slots, double* grad_coef, double
   a.dp_in = A.n_in ? A.dp_weight / (3.0 * T * A.n_in) : 0.0;
   a.dp_out = A.n_out ? A.dp_weight / (3.0 * T * A.n_out) : 0.0;
   a.obj_w = A.obj_w;
+  a.obj_mask = A.obj_mask;
   a.n_tgt = W.n_tgt;
   a.tgt = W.tgt;
   a.tgt_src = W.tgt_src;

@@ -509,21 +509,35 @@ def main():
 
     # Precision study over the workload's own horizon (8000 steps from rest),
     # whatever --steps says. Each run is one batch with HIP-event timing, so
-    # the FP32 run is also the sustained measurement of the step kernel.
+    # the FP32 run is also the sustained measurement of the step kernel. The
+    # fields compared against FP64 are read right after each run.
     study = {}
     for name in names:
         s, wall, ev_ms, step_ms, flush_ms = run_precision(geom, basis, precs[name], args.precision_steps, 0)
-        study[name] = dict(solver=s, wall=wall, step_ms=step_ms, flush_ms=flush_ms,
+        w, _, m = s.wss()
+        _, u = s.macroscopic()
+        study[name] = dict(solver=s, wall=wall, step_ms=step_ms, flush_ms=flush_ms, wss=w, tawss=m, vel=u,
                            mlups=N * args.precision_steps / wall / 1e6)
         print(f"# {name}: {args.precision_steps} steps, {study[name]['mlups']:.0f} MLUPS, step kernel "
               f"{step_ms * 1e3 / args.precision_steps:.2f} us", file=sys.stderr, flush=True)
-
-    # The timed region of the bench contract: W warm-up steps, K timed steps.
-    s, wall, ev_ms, step_ms, flush_ms = run_precision(geom, basis, lbm.PREC_F32, args.steps, args.warmup)
-    sinfo = s.info()
-    s.close()
-    value = N * args.steps / wall / 1e6
-    print(f"# timed region: {args.steps} FP32 steps, {value:.0f} MLUPS", file=sys.stderr, flush=True)
+        if name == "f32":
+            # The timed region of the bench contract (W warm-up steps, K timed
+            # steps) on the same solver, returned to rest: a second FP32
+            # solver made later in the process has run its cached-input steps
+            # at 78-87 us instead of 62-71 us (the Infinity Cache holding less
+            # of its state), the first one has not.
+            s.set_populations(None)
+            if args.warmup:
+                s.step_coefficients(coefficients(0, args.warmup))
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step_coefficients(coefficients(args.warmup, args.steps))
+            s.synchronize()
+            timed_wall = time.perf_counter() - t0
+            timed_step_ms, _ = s.last_batch_kernel_ms()
+            sinfo = s.info()
+            value = N * args.steps / timed_wall / 1e6
+            print(f"# timed region: {args.steps} FP32 steps, {value:.0f} MLUPS", file=sys.stderr, flush=True)
 
     copy_gbps = lbm.copy_probe(1 << 30, iters=20)
     print(f"# copy probe: {copy_gbps:.0f} GB/s (1 GiB read + write)", file=sys.stderr, flush=True)
@@ -532,11 +546,11 @@ def main():
     # time less the event-timed stress-flush launches (one per 8 steps).
     sus = study["f32"]
     per_launch_s = sus["step_ms"] / 1e3 / args.precision_steps
-    burst_launch_s = step_ms / 1e3 / args.steps
+    burst_launch_s = timed_step_ms / 1e3 / args.steps
     achieved = bytes_per_launch / per_launch_s / 1e9
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "warmup": args.warmup, "ms_per_step": round(timed_wall * 1e3 / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "precision-study Y-bifurcation (parent radius 24, daughters 18 at 35 degrees), "
@@ -559,14 +573,11 @@ def main():
     if "f64" in study:
         out["fp64_mlups"] = round(study["f64"]["mlups"], 1)
         out["fp32_over_fp64"] = round(sus["mlups"] / study["f64"]["mlups"], 3)
-        ref = study["f64"]["solver"]
-        w64, _, m64 = ref.wss()
-        _, u64 = ref.macroscopic()
+        w64, m64, u64 = study["f64"]["wss"], study["f64"]["tawss"], study["f64"]["vel"]
         for name in ("f32", "f16"):
             if name not in study:
                 continue
-            w, _, m = study[name]["solver"].wss()
-            _, u = study[name]["solver"].macroscopic()
+            w, m, u = study[name]["wss"], study[name]["tawss"], study[name]["vel"]
             out[f"{name}_wss_rel_l2"] = rel_l2(w, w64)
             out[f"{name}_tawss_rel_l2"] = rel_l2(m, m64)
             out[f"{name}_velocity_rel_l2"] = rel_l2(u, u64)

@@ -184,36 +184,34 @@ def run_adjoint(geom, info, steps=ADJ_STEPS):
     case = adjoint_case(geom, info)
     wave = case["wave"][:steps]
     coef = lambda th: wave[:, None] * th[None, :]  # noqa: E731
-    # Objective weights: the stress term scaled to the size of the pressure term.
-    # over the whole run: the segment lies ~900 lu downstream of the inlet,
-    # which the flow reaches only after ~1600 steps
-    cal = adjoint_solver(geom, case, lbm.PREC_F32)
-    jw = cal.adjoint_forward(steps, coef(case["theta"]), wss_weight=1.0, dp_weight=0.0,
-                             wss_mask=case["mask"], checkpoint_every=ADJ_EVERY)
-    cal.close()
-    cal = adjoint_solver(geom, case, lbm.PREC_F32)
-    jp = cal.adjoint_forward(steps, coef(case["theta"]), wss_weight=0.0, dp_weight=1.0, checkpoint_every=ADJ_EVERY)
-    cal.close()
-    w_wss = abs(jp / jw) if jw > 0 else 1.0
-    obj = dict(wss_weight=w_wss, dp_weight=1.0, wss_mask=case["mask"], checkpoint_every=ADJ_EVERY)
 
     def timed(fn):
         t0 = time.perf_counter()
         r = fn()
         return r, time.perf_counter() - t0
 
-    fwd = adjoint_solver(geom, case, lbm.PREC_F32)
-    c = coef(case["theta"])
-    fwd.step_coefficients(c[:10])  # warm the kernels
-    fwd.close()
-    fwd = adjoint_solver(geom, case, lbm.PREC_F32)
-    _, t_fwd = timed(lambda: (fwd.step_coefficients(c), fwd.synchronize()))
-    fwd.close()
-    # The gradient as an optimisation loop sees it: the second forward/backward
-    # pair on the solver (the first allocates the checkpoint arena and the
-    # reverse pass's workspace, which later pairs reuse; lbm_set_populations
-    # (NULL) returns the solver to its rest state in between).
+    # Every FP32 pass below runs on one solver, returned to rest between them
+    # (lbm_set_populations(NULL)), so that the forward pass and the gradient
+    # are timed on the same state buffers: how much of a solver's state the
+    # Infinity Cache holds has differed between solvers of one process.
     a32 = adjoint_solver(geom, case, lbm.PREC_F32)
+    c = coef(case["theta"])
+    # Objective weights: the stress term scaled to the size of the pressure
+    # term over the whole run (the segment lies ~900 lu downstream of the
+    # inlet, which the flow reaches only after ~1600 steps).
+    jw = a32.adjoint_forward(steps, c, wss_weight=1.0, dp_weight=0.0, wss_mask=case["mask"],
+                             checkpoint_every=ADJ_EVERY)
+    a32.set_populations(None)
+    jp = a32.adjoint_forward(steps, c, wss_weight=0.0, dp_weight=1.0, checkpoint_every=ADJ_EVERY)
+    a32.set_populations(None)
+    w_wss = abs(jp / jw) if jw > 0 else 1.0
+    obj = dict(wss_weight=w_wss, dp_weight=1.0, wss_mask=case["mask"], checkpoint_every=ADJ_EVERY)
+    # The forward pass alone.
+    _, t_fwd = timed(lambda: (a32.step_coefficients(c), a32.synchronize()))
+    a32.set_populations(None)
+    # The gradient as an optimisation loop sees it: the second forward/backward
+    # pair (the first allocates the reverse pass's workspace, which later pairs
+    # reuse).
     a32.adjoint_forward(steps, c, **obj)
     a32.adjoint_backward(0)
     a32.set_populations(None)
@@ -241,8 +239,8 @@ def run_adjoint(geom, info, steps=ADJ_STEPS):
                     f"{ADJ_MODES} Zernike inlet coefficients + per-node Brinkman porosity, "
                     f"J = mean sum |WSS|^2 over a {ADJ_SEGMENT_LEN:g}-lu daughter segment "
                     f"({int(case['mask'].sum())} near-wall nodes) + pressure drop",
-        "timing": "forward: a fresh solver's 4000 steps; adjoint: the second checkpointed forward + reverse "
-                  "pair of one solver (the first pair allocates the arena and workspace)",
+        "timing": "one FP32 solver returned to rest between passes: its forward pass of 4000 steps, and its "
+                  "second checkpointed forward + reverse pair (the first allocates the reverse workspace)",
         "forward_s": round(t_fwd, 4), "adjoint_forward_s": round(t_af, 4), "backward_s": round(t_bw, 4),
         "adjoint_over_forward": round(t_bw / t_fwd, 3), "gradient_over_forward": round((t_af + t_bw) / t_fwd, 3),
         "J_f32": J32, "J_f64": J64, "J_wss_unit_weight": jw, "J_pressure_drop": jp, "wss_weight": w_wss,

@@ -157,6 +157,10 @@ __device__ inline int32_t main_block_of(const ColArgs& a, int32_t b) {
 }
 
 hipError_t launch_columns(int Q, int precision, int kmode, bool force, const ColArgs& a, hipStream_t st);
+// Device copy of `bytes` (a multiple of 16) with nontemporal loads and stores:
+// state copies that are not read again soon (checkpoints) pass the caches
+// by, instead of taking the Infinity Cache room of a cache-resident state.
+hipError_t launch_stream_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
 // Adds ring slots 0 .. count-1 (in order) to the FP64 accumulators and sets
 // the instantaneous |tau| from slot count-1.
 hipError_t launch_wss_flush(int precision, const ColArgs& a, int32_t count, hipStream_t st);

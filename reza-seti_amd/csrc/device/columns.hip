@@ -218,6 +218,26 @@ hipError_t launch_wss_flush(int precision, const ColArgs& a, int32_t count, hipS
   return hipGetLastError();
 }
 
+namespace {
+// One 16-byte element per thread, nontemporal both ways.
+__global__ __launch_bounds__(BLOCK) void stream_copy_kernel(const void* src, void* dst, int64_t n16) {
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i < n16)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(static_cast<const V4*>(src) + i), static_cast<V4*>(dst) + i);
+}
+}  // namespace
+
+hipError_t launch_stream_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes % 16) return hipErrorInvalidValue;
+  const int64_t n16 = static_cast<int64_t>(bytes / 16);
+  if (n16 == 0) return hipSuccess;
+  const int64_t grid = (n16 + BLOCK - 1) / BLOCK;
+  if (grid >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(static_cast<unsigned>(grid)), dim3(BLOCK), 0, st, src, dst, n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_columns_wss_stats(const ColArgs& a, double* partials, double* out2, hipStream_t st) {
   const int32_t nb = (a.n_nw + BLOCK - 1) / BLOCK;
   if (nb == 0) return hipMemsetAsync(out2, 0, 2 * sizeof(double), st);

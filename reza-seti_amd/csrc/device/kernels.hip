@@ -383,9 +383,11 @@ __global__ __launch_bounds__(BLOCK) void finish_kernel(const double* partials, i
 }
 
 // The two launches of the FP64 step (launch_split); together they write
-// what one lbm_kernel launch writes.
+// what one lbm_kernel launch writes. The special launch is bounded to 4 waves
+// per SIMD (128 VGPRs instead of 168, some spills on the heavier paths):
+// 355 -> 345 us per step on the 2.85M-node bifurcation.
 template <class L, class S, int MODE, bool FORCE, bool NT>
-__global__ __launch_bounds__(BLOCK) void lbm_special_kernel(StepArgs a) {
+__global__ __attribute__((amdgpu_waves_per_eu(4))) __launch_bounds__(BLOCK) void lbm_special_kernel(StepArgs a) {
   const int32_t p = blockIdx.x * BLOCK + threadIdx.x;
   if (p < a.n_special_pad) Kern<L, S>::template special_node<MODE, FORCE, NT>(a, p);
 }

@@ -783,7 +783,8 @@ int create_columns(lbm_solver* s, const Tables& t) {
   // ones: FP32 step 86.8 -> 82.1 us, FP64 +6.6%; runs of 16 or 256 blocks
   // gained less or lost). A state whose two buffers both fit the Infinity
   // Cache keeps the contiguous eighths (FP16 on that vessel: 2% faster).
-  c.xcd_run = s->cache_policy == 0 ? 0 : 64;
+  // FP64 takes runs of 32 blocks (179.0 against 180.2 us on that vessel).
+  c.xcd_run = s->cache_policy == 0 ? 0 : (s->prec == 0 ? 32 : 64);
   c.cache_hint = CACHE_PLAIN;
 
   // Generic fields used by the shared moments kernel and init_state.
@@ -1516,7 +1517,7 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
     A.prev = W.ckpt_arena + static_cast<size_t>(A.nck) * state_bytes;
   }
   A.coef.assign(static_cast<size_t>(T) * M, 0.0);
-  HIP_TRY(hipMemcpyAsync(A.ckpt[0], s->fs[s->cur], state_bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(launch_stream_copy(A.ckpt[0], s->fs[s->cur], state_bytes, s->stream));
   const bool force = has_force(s);
   for (int32_t t = 0; t < T; ++t) {
     ColArgs c = col_args_for(s, s->cur);
@@ -1539,10 +1540,9 @@ int lbm_adjoint_forward(lbm_solver* s, int32_t nsteps, const double* coef, const
     if (s->prm.flow_type == LBM_FLOW_PULSATILE) s->total_runtime += s->prm.dt;
     s->stepped = true;
     if ((t + 1) % K == 0 || t + 1 == T)
-      HIP_TRY(hipMemcpyAsync(A.ckpt[(t + 1 + K - 1) / K], s->fs[s->cur], state_bytes, hipMemcpyDeviceToDevice,
-                             s->stream));
+      HIP_TRY(launch_stream_copy(A.ckpt[(t + 1 + K - 1) / K], s->fs[s->cur], state_bytes, s->stream));
   }
-  HIP_TRY(hipMemcpyAsync(A.prev, s->fs[1 - s->cur], state_bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(launch_stream_copy(A.prev, s->fs[1 - s->cur], state_bytes, s->stream));
   std::vector<double> acc(nnw), rho(nb);
   HIP_TRY(hipMemcpyAsync(acc.data(), A.obj_acc, nnw * 8, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipMemcpyAsync(rho.data(), A.obj_rho, nb * 8, hipMemcpyDeviceToHost, s->stream));

@@ -45,6 +45,13 @@ struct Kern {
   // Missing links take the slot's previous post-BC value (reference: the
   // target of a push that never happens keeps its value, Simulation.cpp:245-251)
   // or, in half-way bounce-back mode, the slot's own opposite population.
+  template <bool NTL = false>
+  __device__ static C ld_pop(const void* base, int64_t i) {
+    if constexpr (NTL) return P::ld_nt(base, i);
+    else return P::ld(base, i);
+  }
+
+  template <bool NTL = false>
   __device__ static void gather(const StepArgs& a, int32_t p, C* F, uint32_t& miss) {
     const bool special = p < a.n_special_pad;
     miss = 0;
@@ -52,7 +59,7 @@ struct Kern {
     for (int q = 0; q < Q; ++q) {
       const int32_t s = (q == L::REST) ? p : a.src[int64_t(q) * a.npad + p];
       if (s >= 0) {
-        C v = P::ld(a.fs_in, int64_t(q) * a.npad + s);
+        C v = ld_pop<NTL>(a.fs_in, int64_t(q) * a.npad + s);
         if (a.zero_check && v == C(0)) {
           // Simulation.cpp:247: a zero f* is not streamed.
           if (special) v = C(a.fprev_in[int64_t(q) * a.n_special_pad + p]);
@@ -159,8 +166,11 @@ struct Kern {
     for (int q = 0; q < Q; ++q) a.f_out[int64_t(q) * a.npad + p] = SHIFT ? double(F[q]) + L::w[q] : double(F[q]);
   }
 
-  // NT: nontemporal population stores (a state larger than the Infinity
-  // Cache is streamed through it).
+  // NT (a state larger than the Infinity Cache): the populations are loaded
+  // and stored nontemporally and so are the special slots' previous values,
+  // so that the int32 pull table, read whole every step, stays in the
+  // Infinity Cache (2.85M-node bifurcation, FP64: 344.5 -> 335.1 us per step
+  // against nontemporal population stores alone).
   template <int MODE, bool FORCE, bool NT = false>
   __device__ static void special_node(const StepArgs& a, int32_t p) {
     const uint32_t fl = a.sflags[p];
@@ -172,7 +182,7 @@ struct Kern {
       load_direct(a, p, F);
     } else {
       uint32_t miss;
-      gather(a, p, F, miss);
+      gather<NT>(a, p, F, miss);
       if (kind == 2 /*inlet*/) {
         if (a.inlet_mode == INLET_ZOUHE) {
           if constexpr (Q == 19) zouhe_ref<C, SHIFT>(F, inlet_speed(a, p), a.inlet_normal);
@@ -224,7 +234,10 @@ struct Kern {
     // Reference post-BC f of boundary nodes, for links that stream nothing.
     if (a.fprev_out) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) a.fprev_out[int64_t(q) * a.n_special_pad + p] = double(F[q]);
+      for (int q = 0; q < Q; ++q) {
+        if constexpr (NT) __builtin_nontemporal_store(double(F[q]), a.fprev_out + int64_t(q) * a.n_special_pad + p);
+        else a.fprev_out[int64_t(q) * a.n_special_pad + p] = double(F[q]);
+      }
     }
     const bool zero_u = (kind == 1) && (a.wall_mode == WALL_REF);
     collide_store<MODE, FORCE, NT>(a, p, F, zero_u, nearwall, true);
@@ -239,7 +252,7 @@ struct Kern {
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int32_t s = (q == L::REST) ? p : a.src[int64_t(q) * a.npad + p];
-        const C v = P::ld(a.fs_in, int64_t(q) * a.npad + s);
+        const C v = ld_pop<NT>(a.fs_in, int64_t(q) * a.npad + s);
         if (!SHIFT && a.zero_check && v == C(0)) atomicOr(a.err_flags, ERRF_ZERO_SKIP_BULK);
         F[q] = v;
       }

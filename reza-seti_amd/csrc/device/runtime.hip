@@ -1040,8 +1040,10 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
   a.outlet_mode = p->outlet_mode;
   a.outlet_rho = p->outlet_rho;
   a.zero_check = (p->parity_checks && p->precision == LBM_PREC_F64 && p->wall_mode == LBM_WALL_REFERENCE) ? 1 : 0;
-  // A state that cannot stay in the Infinity Cache is streamed past it
-  // (2.85M-node bifurcation, FP64: 367 -> 355 us per step).
+  // A state that cannot stay in the Infinity Cache is streamed past it, and
+  // the pull table stays there (2.85M-node bifurcation, FP64: 367 us per step
+  // with plain accesses, 355 with nontemporal population stores, 335 with
+  // nontemporal population loads and previous-value stores as well).
   a.nt_stores = pop_bytes > kInfinityCacheBytes ? 1 : 0;
   a.err_flags = s->err;
 

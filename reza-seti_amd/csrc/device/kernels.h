@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "columns.h"
+
 namespace lbm {
 
 // Per-slot flags of the special-node range.
@@ -31,6 +33,11 @@ struct StepArgs {
   const void* fs_in;
   void* fs_out;
   const int32_t* src;  // pull table q * npad + slot: slot of x - c_q, or -1
+  // Bulk slots as column segments: one ChunkMeta per 64 slots from
+  // n_special_pad (seg_first, seg_mask; irr_mask: the slot reads its row of
+  // `src`) and NCOL deltas per segment; null: every slot reads `src`.
+  const ChunkMeta* bmeta;
+  const int32_t* bdelta;
   int64_t npad;
   // Slot ranges: special nodes [0, n_special), padding up to n_special_pad,
   // bulk nodes [n_special_pad, bulk_end).

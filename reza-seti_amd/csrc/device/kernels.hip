@@ -243,16 +243,47 @@ struct Kern {
     collide_store<MODE, FORCE, NT>(a, p, F, zero_u, nearwall, true);
   }
 
+  // Pull sources of bulk slot p: p + delta(o) - c_z from its column
+  // segment (p - c_z in its own column), or its row of the pull table where
+  // the chunk marks the slot irregular (the host checked every regular slot's
+  // decode against the table). The branch keeps the table loads off the
+  // regular lanes.
+  __device__ static void bulk_sources(const StepArgs& a, int32_t p, int32_t* src) {
+    bool row = true;
+    int4 lo = make_int4(0, 0, 0, 0), hi = lo;
+    if (a.bmeta) {
+      const int32_t c = __builtin_amdgcn_readfirstlane((p - a.n_special_pad) / CHUNK);
+      const ChunkMeta m = a.bmeta[c];
+      const int l = p & (CHUNK - 1);
+      row = (m.irr_mask >> l) & 1u;
+      const int4* d4 = reinterpret_cast<const int4*>(a.bdelta) + 2 * int64_t(segment_of(m, l));
+      lo = d4[0];
+      hi = d4[1];
+    }
+    if (row) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) src[q] = (q == L::REST) ? p : a.src[int64_t(q) * a.npad + p];
+    } else {
+      const int32_t d[NCOL] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int o = col_of(L::cx[q], L::cy[q]);
+        src[q] = (o < 0 ? p : p + d[o]) - L::cz[q];
+      }
+    }
+  }
+
   template <int MODE, bool FORCE, bool NT = false>
   __device__ static void bulk_node(const StepArgs& a, int32_t p) {
     C F[Q];
     if (MODE == KMODE_INIT) {
       load_direct(a, p, F);
     } else {
+      int32_t src[Q];
+      bulk_sources(a, p, src);
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
-        const int32_t s = (q == L::REST) ? p : a.src[int64_t(q) * a.npad + p];
-        const C v = ld_pop<NT>(a.fs_in, int64_t(q) * a.npad + s);
+        const C v = ld_pop<NT>(a.fs_in, int64_t(q) * a.npad + src[q]);
         if (!SHIFT && a.zero_check && v == C(0)) atomicOr(a.err_flags, ERRF_ZERO_SKIP_BULK);
         F[q] = v;
       }

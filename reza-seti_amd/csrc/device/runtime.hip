@@ -799,6 +799,100 @@ int create_columns(lbm_solver* s, const Tables& t) {
   return LBM_OK;
 }
 
+// Column segments of the reference-rule family's bulk slots [n_special_pad,
+// bulk_end): plain fluid nodes in reference order, so that runs of slots
+// whose nodes share (i, j) with consecutive k are column segments as in the
+// column family, split where a slot's links disagree with the deltas the
+// segment has established (its neighbour column changed, or a neighbour is a
+// special slot, which sits in the leading range). One ChunkMeta per 64 bulk
+// slots carries seg_first, seg_mask and irr_mask; irr_mask marks every slot
+// for which the kernel's decode (lane -> segment from the masks, then
+// p + delta(o) - c_z, or p - c_z in its own column) does not give its row of
+// the pull table in every direction, and those slots read their row instead.
+// The decode is replayed here against the table for every slot, so a regular
+// slot's sources are its table entries.
+void bulk_segments(const lbm_solver* s, const Tables& t, const std::vector<int32_t>& src, int Q,
+                   std::vector<ChunkMeta>& meta, std::vector<int32_t>& delta) {
+  const lbm::LatticeView L = lbm::lattice_view(Q);
+  const int64_t npad = s->npad, sp = s->n_special_pad, nb = s->bulk_end - s->n_special_pad;
+  const int64_t nchunk = (nb + CHUNK - 1) / CHUNK;
+  std::vector<int32_t> seg(static_cast<size_t>(nb));
+  delta.clear();
+  int32_t cur[NCOL] = {};
+  bool have[NCOL] = {};
+  for (int64_t i = 0; i < nb; ++i) {
+    const int64_t p = sp + i;
+    bool split = i == 0;
+    if (!split) {
+      const int32_t* c = &t.ijk[3 * static_cast<size_t>(s->node_of[p])];
+      const int32_t* b = &t.ijk[3 * static_cast<size_t>(s->node_of[p - 1])];
+      split = c[0] != b[0] || c[1] != b[1] || c[2] != b[2] + 1;
+    }
+    bool ok = true;
+    int32_t nd[NCOL] = {};
+    bool nh[NCOL] = {};
+    for (int q = 0; q < Q && ok; ++q) {
+      const int32_t sq = src[static_cast<size_t>(q) * npad + p];
+      if (sq < 0) {
+        ok = false;
+        break;
+      }
+      const int o = col_of(L.cx[q], L.cy[q]);
+      if (o < 0) {
+        if (int64_t(sq) != p - L.cz[q]) ok = false;
+        continue;
+      }
+      const int32_t dl = static_cast<int32_t>(int64_t(sq) + L.cz[q] - p);
+      if (nh[o] && nd[o] != dl) ok = false;
+      nd[o] = dl;
+      nh[o] = true;
+    }
+    if (ok && !split)
+      for (int o = 0; o < NCOL; ++o)
+        if (nh[o] && have[o] && cur[o] != nd[o]) split = true;
+    if (split) {
+      delta.insert(delta.end(), NCOL, 0);
+      std::fill(have, have + NCOL, false);
+    }
+    if (ok)
+      for (int o = 0; o < NCOL; ++o)
+        if (nh[o] && !have[o]) {
+          cur[o] = nd[o];
+          have[o] = true;
+          delta[delta.size() - NCOL + o] = nd[o];
+        }
+    seg[static_cast<size_t>(i)] = static_cast<int32_t>(delta.size() / NCOL - 1);
+  }
+  if (delta.empty()) delta.assign(NCOL, 0);
+  meta.assign(static_cast<size_t>(std::max<int64_t>(nchunk, 1)), ChunkMeta{});
+  for (int64_t c = 0; c < nchunk; ++c) {
+    ChunkMeta& m = meta[c];
+    m.seg_first = seg[static_cast<size_t>(c * CHUNK)];
+    for (int l = 0; l < CHUNK; ++l) {
+      const int64_t i = c * CHUNK + l;
+      const uint64_t bit = uint64_t(1) << l;
+      if (i >= nb) {
+        m.irr_mask |= bit;  // past bulk_end: never stepped
+        continue;
+      }
+      if (l > 0 && seg[i] != seg[i - 1]) m.seg_mask |= bit;
+    }
+    for (int l = 0; l < CHUNK; ++l) {
+      const int64_t i = c * CHUNK + l;
+      if (i >= nb) break;
+      const int64_t p = sp + i;
+      const int64_t sg = segment_of(m, l);
+      bool reg = sg == seg[static_cast<size_t>(i)];
+      for (int q = 0; q < Q && reg; ++q) {
+        const int o = col_of(L.cx[q], L.cy[q]);
+        const int64_t dec = (o < 0 ? p : p + delta[static_cast<size_t>(sg) * NCOL + o]) - L.cz[q];
+        if (dec != src[static_cast<size_t>(q) * npad + p]) reg = false;
+      }
+      if (!reg) m.irr_mask |= uint64_t(1) << l;
+    }
+  }
+}
+
 // Device buffers of one reverse pass, released on every exit path.
 struct AdjointScratch {
   std::vector<void*> bufs;
@@ -953,11 +1047,16 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
   for (size_t i = 0; i < t.nearwall.size(); ++i)
     for (int a = 0; a < 3; ++a) nrm[a * sp + s->slot_of[t.nearwall[i]]] = t.normals[3 * i + a];
 
+  std::vector<ChunkMeta> bmeta;
+  std::vector<int32_t> bdelta;
+  bulk_segments(s, t, src, Q, bmeta, bdelta);
+
   // Device buffers.
   int rc = LBM_OK;
   const size_t pop_bytes = static_cast<size_t>(Q) * npad * s->esize;
   if ((rc = dalloc_raw(s, &s->fs[0], pop_bytes)) || (rc = dalloc_raw(s, &s->fs[1], pop_bytes)) ||
-      (rc = dalloc(s, &s->src, src.size())) || (rc = dalloc(s, &s->sflags, sp)) || (rc = dalloc(s, &s->bc_nbr, sp)) ||
+      (rc = dalloc(s, &s->src, src.size())) || (rc = dalloc(s, &s->meta, bmeta.size())) ||
+      (rc = dalloc(s, &s->seg_delta, bdelta.size())) || (rc = dalloc(s, &s->sflags, sp)) || (rc = dalloc(s, &s->bc_nbr, sp)) ||
       (rc = dalloc(s, &s->inlet_slot, sp)) || (rc = dalloc(s, &s->normal, 3 * sp)) ||
       (rc = dalloc(s, &s->rho_u, 4 * sp)) || (rc = dalloc(s, &s->wss, 5 * sp)) || (rc = dalloc(s, &s->err, 1)))
     return bail(rc);
@@ -978,7 +1077,8 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
   auto up = [&](void* d, const void* h, size_t bytes) {
     return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s->stream) : hipSuccess;
   };
-  if (up(s->src, src.data(), src.size() * 4) || up(s->sflags, sflags.data(), sp * 4) ||
+  if (up(s->src, src.data(), src.size() * 4) || up(s->meta, bmeta.data(), bmeta.size() * sizeof(ChunkMeta)) ||
+      up(s->seg_delta, bdelta.data(), bdelta.size() * 4) || up(s->sflags, sflags.data(), sp * 4) ||
       up(s->bc_nbr, bcn.data(), sp * 4) || up(s->inlet_slot, islot.data(), sp * 4) ||
       up(s->normal, nrm.data(), nrm.size() * 8) || up(s->inlet_basis, basis.data(), basis.size() * 8) ||
       hipMemsetAsync(s->rho_u, 0, 4 * sp * 8, s->stream) || hipMemsetAsync(s->wss, 0, 5 * sp * 8, s->stream) ||
@@ -990,6 +1090,8 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
   StepArgs& a = s->base;
   std::memset(&a, 0, sizeof(a));
   a.src = s->src;
+  a.bmeta = s->meta;
+  a.bdelta = s->seg_delta;
   a.npad = npad;
   a.n_special = static_cast<int32_t>(s->n_special);
   a.n_special_pad = static_cast<int32_t>(sp);

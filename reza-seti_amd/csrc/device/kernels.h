@@ -33,9 +33,9 @@ struct StepArgs {
   const void* fs_in;
   void* fs_out;
   const int32_t* src;  // pull table q * npad + slot: slot of x - c_q, or -1
-  // Bulk slots as column segments: one ChunkMeta per 64 slots from
-  // n_special_pad (seg_first, seg_mask; irr_mask: the slot reads its row of
-  // `src`) and NCOL deltas per segment; null: every slot reads `src`.
+  // Slots as column segments: one ChunkMeta per 64 slots (seg_first,
+  // seg_mask; irr_mask: the slot reads its row of `src`) and NCOL deltas per
+  // segment; null: every slot reads `src`.
   const ChunkMeta* bmeta;
   const int32_t* bdelta;
   int64_t npad;

@@ -51,13 +51,23 @@ struct Kern {
     else return P::ld(base, i);
   }
 
-  template <bool NTL = false>
+  // OWN: p is the thread's own slot (wave-uniform chunk), so its sources may
+  // come from the column segments; otherwise (the node behind an
+  // inlet/outlet node) from the table.
+  template <bool NTL = false, bool OWN = false>
   __device__ static void gather(const StepArgs& a, int32_t p, C* F, uint32_t& miss) {
     const bool special = p < a.n_special_pad;
     miss = 0;
+    int32_t src[Q];
+    if constexpr (OWN) {
+      slot_sources(a, p, src);
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) src[q] = (q == L::REST) ? p : a.src[int64_t(q) * a.npad + p];
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int32_t s = (q == L::REST) ? p : a.src[int64_t(q) * a.npad + p];
+      const int32_t s = src[q];
       if (s >= 0) {
         C v = ld_pop<NTL>(a.fs_in, int64_t(q) * a.npad + s);
         if (a.zero_check && v == C(0)) {
@@ -182,7 +192,7 @@ struct Kern {
       load_direct(a, p, F);
     } else {
       uint32_t miss;
-      gather<NT>(a, p, F, miss);
+      gather<NT, true>(a, p, F, miss);
       if (kind == 2 /*inlet*/) {
         if (a.inlet_mode == INLET_ZOUHE) {
           if constexpr (Q == 19) zouhe_ref<C, SHIFT>(F, inlet_speed(a, p), a.inlet_normal);
@@ -243,16 +253,16 @@ struct Kern {
     collide_store<MODE, FORCE, NT>(a, p, F, zero_u, nearwall, true);
   }
 
-  // Pull sources of bulk slot p: p + delta(o) - c_z from its column
-  // segment (p - c_z in its own column), or its row of the pull table where
-  // the chunk marks the slot irregular (the host checked every regular slot's
-  // decode against the table). The branch keeps the table loads off the
-  // regular lanes.
-  __device__ static void bulk_sources(const StepArgs& a, int32_t p, int32_t* src) {
+  // Pull sources of slot p (wave-uniform chunk): p + delta(o) - c_z from its
+  // column segment (p - c_z in its own column), or its row of the pull table
+  // (-1 for a missing link) where the chunk marks the slot irregular (the host
+  // checked every regular slot's decode against the table). The branch keeps
+  // the table loads off the regular lanes.
+  __device__ static void slot_sources(const StepArgs& a, int32_t p, int32_t* src) {
     bool row = true;
     int4 lo = make_int4(0, 0, 0, 0), hi = lo;
     if (a.bmeta) {
-      const int32_t c = __builtin_amdgcn_readfirstlane((p - a.n_special_pad) / CHUNK);
+      const int32_t c = __builtin_amdgcn_readfirstlane(p / CHUNK);
       const ChunkMeta m = a.bmeta[c];
       const int l = p & (CHUNK - 1);
       row = (m.irr_mask >> l) & 1u;
@@ -280,7 +290,7 @@ struct Kern {
       load_direct(a, p, F);
     } else {
       int32_t src[Q];
-      bulk_sources(a, p, src);
+      slot_sources(a, p, src);
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const C v = ld_pop<NT>(a.fs_in, int64_t(q) * a.npad + src[q]);

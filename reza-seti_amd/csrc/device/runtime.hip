@@ -799,36 +799,36 @@ int create_columns(lbm_solver* s, const Tables& t) {
   return LBM_OK;
 }
 
-// Column segments of the reference-rule family's bulk slots [n_special_pad,
-// bulk_end): plain fluid nodes in reference order, so that runs of slots
-// whose nodes share (i, j) with consecutive k are column segments as in the
-// column family, split where a slot's links disagree with the deltas the
-// segment has established (its neighbour column changed, or a neighbour is a
-// special slot, which sits in the leading range). One ChunkMeta per 64 bulk
-// slots carries seg_first, seg_mask and irr_mask; irr_mask marks every slot
-// for which the kernel's decode (lane -> segment from the masks, then
-// p + delta(o) - c_z, or p - c_z in its own column) does not give its row of
-// the pull table in every direction, and those slots read their row instead.
-// The decode is replayed here against the table for every slot, so a regular
-// slot's sources are its table entries.
-void bulk_segments(const lbm_solver* s, const Tables& t, const std::vector<int32_t>& src, int Q,
+// Column segments of the reference-rule family's slots [0, bulk_end) in slot
+// order: runs of slots whose nodes share (i, j) with consecutive k (plain
+// fluid in reference order after the special groups, and within each special
+// group its nodes in reference order), split where a slot's links disagree
+// with the deltas the segment has established (its neighbour column changed,
+// or a neighbour sits in another group). One ChunkMeta per 64 slots carries
+// seg_first, seg_mask and irr_mask; irr_mask marks every slot for which the
+// kernel's decode (lane -> segment from the masks, then p + delta(o) - c_z,
+// or p - c_z in its own column) does not give its row of the pull table in
+// every direction (a missing link, a padding slot), and those slots read
+// their row instead. The decode is replayed here against the table for every
+// slot, so a regular slot's sources are its table entries.
+void slot_segments(const lbm_solver* s, const Tables& t, const std::vector<int32_t>& src, int Q,
                    std::vector<ChunkMeta>& meta, std::vector<int32_t>& delta) {
   const lbm::LatticeView L = lbm::lattice_view(Q);
-  const int64_t npad = s->npad, sp = s->n_special_pad, nb = s->bulk_end - s->n_special_pad;
-  const int64_t nchunk = (nb + CHUNK - 1) / CHUNK;
-  std::vector<int32_t> seg(static_cast<size_t>(nb));
+  const int64_t npad = s->npad, ns = s->bulk_end;
+  const int64_t nchunk = (ns + CHUNK - 1) / CHUNK;
+  std::vector<int32_t> seg(static_cast<size_t>(ns));
   delta.clear();
   int32_t cur[NCOL] = {};
   bool have[NCOL] = {};
-  for (int64_t i = 0; i < nb; ++i) {
-    const int64_t p = sp + i;
-    bool split = i == 0;
+  for (int64_t p = 0; p < ns; ++p) {
+    const int32_t n = s->node_of[p];
+    bool split = p == 0 || n < 0 || s->node_of[p - 1] < 0;
     if (!split) {
-      const int32_t* c = &t.ijk[3 * static_cast<size_t>(s->node_of[p])];
+      const int32_t* c = &t.ijk[3 * static_cast<size_t>(n)];
       const int32_t* b = &t.ijk[3 * static_cast<size_t>(s->node_of[p - 1])];
       split = c[0] != b[0] || c[1] != b[1] || c[2] != b[2] + 1;
     }
-    bool ok = true;
+    bool ok = n >= 0;
     int32_t nd[NCOL] = {};
     bool nh[NCOL] = {};
     for (int q = 0; q < Q && ok; ++q) {
@@ -861,7 +861,7 @@ void bulk_segments(const lbm_solver* s, const Tables& t, const std::vector<int32
           have[o] = true;
           delta[delta.size() - NCOL + o] = nd[o];
         }
-    seg[static_cast<size_t>(i)] = static_cast<int32_t>(delta.size() / NCOL - 1);
+    seg[static_cast<size_t>(p)] = static_cast<int32_t>(delta.size() / NCOL - 1);
   }
   if (delta.empty()) delta.assign(NCOL, 0);
   meta.assign(static_cast<size_t>(std::max<int64_t>(nchunk, 1)), ChunkMeta{});
@@ -869,20 +869,19 @@ void bulk_segments(const lbm_solver* s, const Tables& t, const std::vector<int32
     ChunkMeta& m = meta[c];
     m.seg_first = seg[static_cast<size_t>(c * CHUNK)];
     for (int l = 0; l < CHUNK; ++l) {
-      const int64_t i = c * CHUNK + l;
+      const int64_t p = c * CHUNK + l;
       const uint64_t bit = uint64_t(1) << l;
-      if (i >= nb) {
+      if (p >= ns) {
         m.irr_mask |= bit;  // past bulk_end: never stepped
         continue;
       }
-      if (l > 0 && seg[i] != seg[i - 1]) m.seg_mask |= bit;
+      if (l > 0 && seg[p] != seg[p - 1]) m.seg_mask |= bit;
     }
     for (int l = 0; l < CHUNK; ++l) {
-      const int64_t i = c * CHUNK + l;
-      if (i >= nb) break;
-      const int64_t p = sp + i;
+      const int64_t p = c * CHUNK + l;
+      if (p >= ns) break;
       const int64_t sg = segment_of(m, l);
-      bool reg = sg == seg[static_cast<size_t>(i)];
+      bool reg = s->node_of[p] >= 0 && sg == seg[static_cast<size_t>(p)];
       for (int q = 0; q < Q && reg; ++q) {
         const int o = col_of(L.cx[q], L.cy[q]);
         const int64_t dec = (o < 0 ? p : p + delta[static_cast<size_t>(sg) * NCOL + o]) - L.cz[q];
@@ -1049,7 +1048,7 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
 
   std::vector<ChunkMeta> bmeta;
   std::vector<int32_t> bdelta;
-  bulk_segments(s, t, src, Q, bmeta, bdelta);
+  slot_segments(s, t, src, Q, bmeta, bdelta);
 
   // Device buffers.
   int rc = LBM_OK;

@@ -49,7 +49,6 @@ struct StepArgs {
   int32_t xcd_remap;
   int32_t xcd_first[8];   // first bulk block index dispatched to XCD x
   int32_t xcd_offset[8];  // bulk chunk offset of XCD x
-  int32_t xcdb_offset[8]; // the same for a launch of the bulk blocks alone (block b on XCD b % 8)
   // Special-node side arrays, index = slot < n_special_pad.
   const uint32_t* sflags;
   const int32_t* bc_nbr;      // slot behind an inlet/outlet node, or -1

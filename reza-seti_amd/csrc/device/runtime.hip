@@ -1108,11 +1108,6 @@ int lbm_create(const lbm_geometry* g, const lbm_params* p, lbm_solver** out) {
       a.xcd_offset[x] = off;
       off += cnt;
     }
-    off = 0;
-    for (int x = 0; x < 8; ++x) {
-      a.xcdb_offset[x] = off;
-      off += x < Bb ? (Bb - 1 - x) / 8 + 1 : 0;
-    }
   }
   a.sflags = s->sflags;
   a.bc_nbr = s->bc_nbr;

@@ -51,8 +51,11 @@ __global__ __launch_bounds__(BLOCK) void columns_kernel(ColArgs a) {
 // inlet/outlet path needs more registers than the main path, and a
 // waves-per-EU bound keeps the main path's occupancy while that rare path
 // spills instead. (FP64 keeps the unbounded kernel: its main path spills.)
+// 6 waves (80 VGPRs, 3 bytes spilled for D3Q19): 77.7 us per FP32 step over
+// 8000 steps of the 2.85M-node bifurcation, against 78.8 us with 7 waves
+// (72 VGPRs, 11 bytes spilled) and 91 VGPRs unbounded.
 #ifndef LBM_COLUMNS_WAVES_PER_EU
-#define LBM_COLUMNS_WAVES_PER_EU 7
+#define LBM_COLUMNS_WAVES_PER_EU 6
 #endif
 #if LBM_COLUMNS_WAVES_PER_EU > 0
 template <class L, class S, int MODE, bool FORCE, bool NT, bool NTL, bool A32, int KRES>

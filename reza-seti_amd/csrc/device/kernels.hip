@@ -41,16 +41,18 @@ struct Kern {
   static constexpr int Q = L::Q;
   static constexpr bool SHIFT = P::shifted;
 
-  // Post-stream populations of slot p (pull), before any boundary rule.
-  // Missing links take the slot's previous post-BC value (reference: the
-  // target of a push that never happens keeps its value, Simulation.cpp:245-251)
-  // or, in half-way bounce-back mode, the slot's own opposite population.
+  // Population load, nontemporal (NTL) when the state is streamed past the
+  // Infinity Cache.
   template <bool NTL = false>
   __device__ static C ld_pop(const void* base, int64_t i) {
     if constexpr (NTL) return P::ld_nt(base, i);
     else return P::ld(base, i);
   }
 
+  // Post-stream populations of slot p (pull), before any boundary rule.
+  // Missing links take the slot's previous post-BC value (reference: the
+  // target of a push that never happens keeps its value, Simulation.cpp:245-251)
+  // or, in half-way bounce-back mode, the slot's own opposite population.
   // OWN: p is the thread's own slot (wave-uniform chunk), so its sources may
   // come from the column segments; otherwise (the node behind an
   // inlet/outlet node) from the table.
